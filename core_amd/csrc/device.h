@@ -1,0 +1,103 @@
+// device.h -- internal device-side structures and kernel launchers.
+#ifndef SPRGPU_DEVICE_H
+#define SPRGPU_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sprgpu_internal.h"
+
+namespace sprg {
+
+// Device view of one mesh part.
+struct MeshView {
+  int64_t nv, nt, ne;
+  int order;         // coordinate-field order 1|2
+  int straight;      // every edge node is the midpoint of its edge
+  const int32_t* tv;   // [nt][4]
+  const int32_t* te;   // [nt][6] or null
+  const double* xyz;   // [nodes][3]
+  const int32_t* v2t_off;  // [nv+1]
+  const int32_t* v2t;      // [4nt]
+  const int32_t* e2t_off;  // [ne+1]
+  const int32_t* e2t;      // [6nt]
+};
+
+// Patch work lists.  Bucket b holds patches whose row count m = q*|set|
+// fits rows_of_bucket(b).
+enum { NBUCKET = 6 };
+__host__ __device__ inline int bucket_rows(int b)
+{
+  const int r[NBUCKET] = {16, 32, 64, 96, 128, 192};
+  return r[b];
+}
+__host__ __device__ inline int bucket_of(int m)
+{
+  for (int b = 0; b < NBUCKET; ++b)
+    if (m <= bucket_rows(b)) return b;
+  return NBUCKET;  // too large
+}
+
+// Status bits written by kernels (read back by the host after a call).
+enum {
+  ST_PATCH_FAIL = 1,     // "all hope is lost"
+  ST_TOO_LARGE = 2,      // a patch exceeded the supported row count
+  ST_XSET_OVERFLOW = 4,  // expanded-set storage too small
+  ST_INTERNAL = 8
+};
+
+struct Lists {
+  int32_t* items[NBUCKET];   // regular: patch ids; expanded: item ids
+  int32_t* count;            // [NBUCKET] regular, [NBUCKET] expanded, [1] expand-queue
+  int32_t* xitems[NBUCKET];
+  int32_t* xq;               // patches queued for expansion
+  // expanded sets
+  int32_t* x_patch;
+  int64_t* x_off;
+  int32_t* x_len;
+  int32_t* x_set;
+  int64_t* x_cursor;         // [0] set entries used, [1] items used
+  int64_t x_cap;             // capacity of x_set
+  int64_t cap;               // capacity of each list (>= number of patches)
+  int32_t* status;
+};
+
+struct FitArgs {
+  MeshView m;
+  int q;           // points per element
+  int nc;          // components
+  int64_t npatch;  // nv (+ ne)
+  const double* ip;
+  double* out;
+  Lists L;
+};
+
+// Launchers (return 0 or an SPRG_E* code).
+int launch_csr(const MeshView& m, int dim, int32_t* off, int32_t* idx,
+               int32_t* tmp, hipStream_t s);
+int launch_scan_i32(int32_t* a, int64_t n, int32_t* work, hipStream_t s);
+int launch_classify(const FitArgs& a, int nterms, hipStream_t s);
+int launch_exact_fit(const FitArgs& a, int nterms, bool expanded, hipStream_t s);
+int launch_expand(const FitArgs& a, int nterms, hipStream_t s);
+int launch_grad(const MeshView& m, int field_order, int nc_in,
+                const double* nodal, int ip_order, double* ip, hipStream_t s);
+
+struct EstArgs {
+  MeshView m;
+  int ip_order, nc;
+  const double* ip;
+  const double* rec;   // recovered field [nodes][nc]
+  int target;          // 0: getSPRSizeField, 1: getTargetSPRSizeField
+  double adapt_ratio;
+  double target_n, alpha, beta;
+  double* elem;        // per tet: [nt] (h*errNorm) or [2nt] (r, h)
+  double* partial;     // per block partial sums [nblocks][2]
+  double* scalars;     // [4]: self, errsum|G, size_factor, spare
+  double* size;        // [nv]
+  int nblocks;
+};
+int launch_estimate(EstArgs& a, hipStream_t s);
+
+}  // namespace sprg
+
+#endif

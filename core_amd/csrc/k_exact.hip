@@ -1,0 +1,470 @@
+// k_exact.hip -- the FP64 patch recovery of spr::recoverField, reproducing the
+// reference's arithmetic operation for operation.
+//
+// The reference fits each patch with mth::decomposeQR / solveFromQR
+// (mth/mthQR.cc:19-163): Householder reflectors from R with sequential
+// column sums, an EXPLICIT m x m Q formed by reflect_rows, y = Q^T b as a
+// sequential dot product, back substitution, and evaluation of the monomials
+// at the entity's node (spr/sprRecoverField.cc:283-353).  Because the size
+// field downstream amplifies a 1-ulp change of the recovered field by ~1e4
+// or more, this form keeps every rounding of that sequence: sums in the same
+// order, no fused multiply-add (the library is built with -ffp-contract=off),
+// correctly rounded sqrt and division.
+//
+// Work mapping (one 64-lane wavefront per block):
+//  - a patch is owned by a group of LANES lanes (16, 32 or 64); a wave runs
+//    64/LANES patches in lock step;
+//  - the R recurrence keeps R (m x NT) and the reflectors V (NT x m) in LDS;
+//    its sequential sums run on one lane per group, its updates on all;
+//  - Q is never stored: every row of Q evolves independently under the
+//    reflectors, so lane j keeps row j in registers (MMAX doubles, static
+//    indexing) and only its first NT entries are kept (all that Q^T b needs
+//    for the back substitution);
+//  - rows of the patch matrix are recomputed from the mesh: sample points
+//    are the mesh-order rule points mapped by the coordinate field.
+#include "device.h"
+#include "spr_math.h"
+
+namespace sprg {
+namespace {
+
+__device__ __forceinline__ double sgn(double x) { return (x < 0) ? -1 : 1; }
+
+__device__ __forceinline__ int64_t node_of(const MeshView& m, int32_t t, int n)
+{
+  return n < 4 ? (int64_t)m.tv[4 * (int64_t)t + n]
+               : m.nv + (int64_t)m.te[6 * (int64_t)t + (n - 4)];
+}
+
+// getSamplePoints (spr/sprRecoverField.cc:193-211): the rule of the MESH
+// order, point l (the N1 rule ignores l), mapped by the coordinate field
+// (apf/apfElement.cc:106-114 summation order).
+__device__ void sample_point(const MeshView& m, int32_t t, int l, double* x)
+{
+  double xi[3], N[10];
+  rule_point(m.order, l, xi);
+  tet_values(m.order, xi, N);
+  int nn = tet_nodes(m.order);
+  x[0] = 0; x[1] = 0; x[2] = 0;
+  for (int ni = 0; ni < nn; ++ni) {
+    const double* p = m.xyz + 3 * node_of(m, t, ni);
+    x[0] += p[0] * N[ni];
+    x[1] += p[1] * N[ni];
+    x[2] += p[2] * N[ni];
+  }
+}
+
+// Rows i = gl, gl+LANES, ... of the monomial matrix A of the tet set S.
+template <int NT, int LANES>
+__device__ void build_rows(const MeshView& m, int q, const int32_t* S, int nrows,
+                           double* R, int gl)
+{
+  for (int i = gl; i < nrows; i += LANES) {
+    double x[3], t[10];
+    sample_point(m, S[i / q], i % q, x);
+    poly_terms(NT == 4 ? 1 : 2, x, t);
+    for (int j = 0; j < NT; ++j) R[i * NT + j] = t[j];
+  }
+}
+
+// Householder R recurrence of mth::decomposeQR restricted to what the solve
+// uses (get_reflector + reflect_columns on columns >= k, mth/mthQR.cc:19-63).
+// R is [m][NT] row-major, V is [NT][MS] (reflector k, zero outside k..m-1).
+// Returns false as soon as a column norm falls below 1e-10 (the reference's
+// rank test, mth/mthQR.cc:31).  Every lane of the block must call it.
+template <int NT, int LANES, int MS>
+__device__ bool r_recurrence(double* R, double* V, double* sDot, double* sNorm,
+                             int m, int gl, bool act)
+{
+  bool ok = act;
+  for (int k = 0; k < NT; ++k) {
+    if (ok && gl == 0) {
+      double cnorm = 0;
+      for (int i = k; i < m; ++i) cnorm += R[i * NT + k] * R[i * NT + k];
+      sNorm[0] = sqrt(cnorm);
+    }
+    __syncthreads();
+    double cn = ok ? sNorm[0] : 0;
+    if (ok && cn < 1e-10) ok = false;
+    if (ok)
+      for (int i = gl; i < MS; i += LANES)
+        V[k * MS + i] = (i >= k && i < m) ? R[i * NT + k] : 0.0;
+    __syncthreads();
+    if (ok && gl == 0) {
+      V[k * MS + k] += sgn(R[k * NT + k]) * cn;
+      double rnorm = 0;
+      for (int i = k; i < m; ++i) rnorm += V[k * MS + i] * V[k * MS + i];
+      sNorm[1] = sqrt(rnorm);
+    }
+    __syncthreads();
+    if (ok) {
+      double rn = sNorm[1];
+      for (int i = k + gl; i < m; i += LANES) V[k * MS + i] /= rn;
+    }
+    __syncthreads();
+    if (ok && gl >= k && gl < NT) {
+      double dot = 0;
+      for (int i = k; i < m; ++i) dot += R[i * NT + gl] * V[k * MS + i];
+      sDot[gl] = dot;
+    }
+    __syncthreads();
+    if (ok)
+      for (int i = k + gl; i < m; i += LANES)
+        for (int j = k; j < NT; ++j) R[i * NT + j] -= 2 * sDot[j] * V[k * MS + i];
+    __syncthreads();
+  }
+  return ok;
+}
+
+__device__ __forceinline__ void patch_set(const FitArgs& a, int64_t p,
+                                          const int32_t** set, int* ns)
+{
+  const MeshView& m = a.m;
+  if (p < m.nv) {
+    *set = m.v2t + m.v2t_off[p];
+    *ns = m.v2t_off[p + 1] - m.v2t_off[p];
+  } else {
+    int64_t e = p - m.nv;
+    *set = m.e2t + m.e2t_off[e];
+    *ns = m.e2t_off[e + 1] - m.e2t_off[e];
+  }
+}
+
+// Sorts patches into row-count buckets; patches with fewer rows than terms
+// go straight to the expansion queue (hasEnoughPoints, :355-363).
+__global__ void k_classify(FitArgs a, int nterms)
+{
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= a.npatch) return;
+  const int32_t* set;
+  int ns;
+  patch_set(a, p, &set, &ns);
+  int m = ns * a.q;
+  if (m < nterms) {
+    int k = atomicAdd(a.L.count + 2 * NBUCKET, 1);
+    a.L.xq[k] = (int32_t)p;
+    return;
+  }
+  int b = bucket_of(m);
+  if (b >= NBUCKET) {
+    atomicOr(a.L.status, ST_TOO_LARGE);
+    return;
+  }
+  int k = atomicAdd(a.L.count + b, 1);
+  a.L.items[b][k] = (int32_t)p;
+}
+
+template <int NT, int MMAX, int LANES, bool EXP>
+__global__ __launch_bounds__(64) void k_exact_fit(FitArgs a, int bucket)
+{
+  constexpr int G = 64 / LANES;
+  __shared__ double sR[G][MMAX * NT];
+  __shared__ double sV[G][NT * MMAX];
+  __shared__ double sRt[G][NT * NT];
+  __shared__ double sY[G][27 * NT];
+  __shared__ double sDot[G][NT];
+  __shared__ double sNorm[G][2];
+  __shared__ int32_t sS[G][MMAX];
+  const int g = threadIdx.x / LANES, gl = threadIdx.x % LANES;
+  const MeshView& m = a.m;
+  const int q = a.q, nc = a.nc;
+  const int32_t* list = EXP ? a.L.xitems[bucket] : a.L.items[bucket];
+  const int nitems = a.L.count[(EXP ? NBUCKET : 0) + bucket];
+  for (int base = blockIdx.x * G; base < nitems; base += gridDim.x * G) {
+    const int item = base + g;
+    const bool act = item < nitems;
+    int64_t p = 0;
+    int ns = 0;
+    const int32_t* set = nullptr;
+    if (act) {
+      int32_t it = list[item];
+      if (EXP) {
+        p = a.L.x_patch[it];
+        set = a.L.x_set + a.L.x_off[it];
+        ns = a.L.x_len[it];
+      } else {
+        p = it;
+        patch_set(a, p, &set, &ns);
+      }
+    }
+    const int mrows = ns * q;
+    for (int i = gl; i < ns; i += LANES) sS[g][i] = set[i];
+    __syncthreads();
+    build_rows<NT, LANES>(m, q, sS[g], mrows, sR[g], gl);
+    __syncthreads();
+    bool ok = r_recurrence<NT, LANES, MMAX>(sR[g], sV[g], sDot[g], sNorm[g], mrows, gl, act);
+    if (act && !ok && gl == 0) {
+      if (EXP) {
+        atomicOr(a.L.status, ST_INTERNAL);
+      } else {
+        int k = atomicAdd(a.L.count + 2 * NBUCKET, 1);
+        a.L.xq[k] = (int32_t)p;
+      }
+    }
+    // keep the top NT x NT of R for the back substitution, reuse sR for Q
+    for (int i = gl; i < NT * NT; i += LANES) sRt[g][i] = sR[g][i];
+    __syncthreads();
+    // Q rows: q_j = e_j, then reflect_rows for k = 0..NT-1 (mth/mthQR.cc:65-80)
+#pragma unroll 1
+    for (int rb = 0; rb < MMAX; rb += LANES) {
+      const int j = rb + gl;
+      double qr[MMAX];
+#pragma unroll
+      for (int l = 0; l < MMAX; ++l) qr[l] = (l == j) ? 1.0 : 0.0;
+      // V is zero outside rows k..m-1, so both loops can run over all MMAX
+      // entries: adding +-0 to the running dot (which starts at +0) and
+      // subtracting +-0 from q leave every value as the reference's loops
+      // from l = k produce it, and the indexing stays static.
+#pragma unroll 1
+      for (int k = 0; k < NT; ++k) {
+        const double* v = sV[g] + k * MMAX;
+        double dot = 0;
+#pragma unroll
+        for (int l = 0; l < MMAX; ++l) dot += qr[l] * v[l];
+        const double t2 = 2 * dot;
+#pragma unroll
+        for (int l = 0; l < MMAX; ++l) qr[l] -= t2 * v[l];
+      }
+      if (ok && j < mrows)
+#pragma unroll
+        for (int i = 0; i < NT; ++i) sR[g][j * NT + i] = qr[i];
+    }
+    __syncthreads();
+    // stage b = IP values of the patch rows in sV when it fits
+    const bool staged = nc <= NT;
+    if (ok && staged)
+      for (int i = gl; i < mrows * nc; i += LANES) {
+        int r = i / nc, c = i % nc;
+        sV[g][i] = a.ip[((int64_t)sS[g][r / q] * q + r % q) * nc + c];
+      }
+    __syncthreads();
+    // y = transpose(Q) b, sequential over rows (mth/mth_def.h:221-232)
+    if (ok)
+      for (int w = gl; w < NT * nc; w += LANES) {
+        int i = w % NT, c = w / NT;
+        double y = 0;
+        for (int r = 0; r < mrows; ++r) {
+          double b = staged ? sV[g][r * nc + c]
+                            : a.ip[((int64_t)sS[g][r / q] * q + r % q) * nc + c];
+          y += sR[g][r * NT + i] * b;
+        }
+        sY[g][c * NT + i] = y;
+      }
+    __syncthreads();
+    // backsubUT (mth/mthQR.cc:131-148) and evalPolynomial at the node
+    if (ok) {
+      double node[3], t[10];
+      for (int j = 0; j < 3; ++j) node[j] = m.xyz[3 * p + j];
+      poly_terms(NT == 4 ? 1 : 2, node, t);
+      for (int c = gl; c < nc; c += LANES) {
+        double x[NT];
+#pragma unroll
+        for (int ii = 0; ii < NT; ++ii) {
+          const int i = NT - ii - 1;
+          x[i] = sY[g][c * NT + i];
+#pragma unroll
+          for (int jj = i + 1; jj < NT; ++jj) x[i] -= sRt[g][i * NT + jj] * x[jj];
+          x[i] /= sRt[g][i * NT + i];
+        }
+        double v = 0;
+#pragma unroll
+        for (int i = 0; i < NT; ++i) v += x[i] * t[i];
+        a.out[p * nc + c] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------
+// Patch expansion (expandAsNecessary, spr/sprRecoverField.cc:165-188,
+// :365-386).  One wave per queued patch.  The set after shared_dim s is
+// every tet that shares at least s+1 vertices with a tet of the set the
+// round started from (a face, an edge or a vertex of a conforming tet
+// mesh), which includes that set; after each step the exact rank test runs.
+
+constexpr int XMAXR = 192;   // rows of a patch the expansion can test
+constexpr int XSET = 192;    // tets in a set
+constexpr int HSIZE = 2048;  // LDS hash of candidate tets
+
+__device__ __forceinline__ int shared_verts(const int32_t* a, const int32_t* b)
+{
+  int s = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) s += (a[i] == b[j]);
+  return s;
+}
+
+template <int NT>
+__global__ __launch_bounds__(64) void k_expand(FitArgs a)
+{
+  __shared__ double sR[XMAXR * NT];
+  __shared__ double sV[NT * XMAXR];
+  __shared__ double sDot[NT];
+  __shared__ double sNorm[2];
+  __shared__ int32_t sCur[256];
+  __shared__ int32_t sOld[XSET];
+  __shared__ int32_t sHash[HSIZE];
+  __shared__ int sN, sOverflow;
+  const int lane = threadIdx.x;
+  const MeshView& m = a.m;
+  const int q = a.q;
+  const int nq = a.L.count[2 * NBUCKET];
+  for (int it = blockIdx.x; it < nq; it += gridDim.x) {
+    const int64_t p = a.L.xq[it];
+    const int32_t* set0;
+    int ns0;
+    patch_set(a, p, &set0, &ns0);
+    if (lane == 0) { sN = ns0; sOverflow = ns0 > XSET; }
+    __syncthreads();
+    if (sOverflow) {
+      if (lane == 0) atomicOr(a.L.status, ST_TOO_LARGE);
+      __syncthreads();
+      continue;
+    }
+    for (int i = lane; i < ns0; i += 64) sCur[i] = set0[i];
+    __syncthreads();
+    bool done = false, failed = false;
+    for (int round = 0; round < 64 && !done && !failed; ++round) {
+      const int nold = sN;
+      for (int i = lane; i < nold; i += 64) sOld[i] = sCur[i];
+      __syncthreads();
+      for (int sd = 2; sd >= 0 && !done; --sd) {
+        for (int i = lane; i < HSIZE; i += 64) sHash[i] = -1;
+        if (lane == 0) sN = 0;
+        __syncthreads();
+        for (int w = lane; w < nold * 4; w += 64) {
+          const int32_t t = sOld[w >> 2];
+          const int32_t* tv = m.tv + 4 * (int64_t)t;
+          const int32_t v = tv[w & 3];
+          for (int32_t k = m.v2t_off[v]; k < m.v2t_off[v + 1]; ++k) {
+            const int32_t u = m.v2t[k];
+            if (shared_verts(m.tv + 4 * (int64_t)u, tv) < sd + 1) continue;
+            // past 256 distinct tets the set is too large anyway; stopping
+            // here keeps the hash far from full so every probe ends
+            if (*(volatile int*)&sN >= 256) break;
+            uint32_t h = ((uint32_t)u * 2654435761u) & (HSIZE - 1);
+            for (;;) {
+              int32_t prev = atomicCAS(&sHash[h], -1, u);
+              if (prev == -1) {
+                int pos = atomicAdd(&sN, 1);
+                if (pos < 256) sCur[pos] = u;
+                break;
+              }
+              if (prev == u) break;
+              h = (h + 1) & (HSIZE - 1);
+            }
+          }
+        }
+        __syncthreads();
+        const int ns = sN;
+        if (ns > XSET) {
+          if (lane == 0) sOverflow = 1;
+          __syncthreads();
+          failed = true;
+          break;
+        }
+        // bitonic sort of sCur[0..256) with padding
+        for (int i = ns + lane; i < 256; i += 64) sCur[i] = 0x7fffffff;
+        __syncthreads();
+        for (int size = 2; size <= 256; size <<= 1)
+          for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = lane; i < 256; i += 64) {
+              int j = i ^ stride;
+              if (j > i) {
+                bool up = (i & size) == 0;
+                int32_t x = sCur[i], y = sCur[j];
+                if ((x > y) == up) { sCur[i] = y; sCur[j] = x; }
+              }
+            }
+            __syncthreads();
+          }
+        const int mrows = ns * q;
+        bool ok = false;
+        if (mrows >= NT && mrows <= XMAXR) {
+          build_rows<NT, 64>(m, q, sCur, mrows, sR, lane);
+          __syncthreads();
+          ok = r_recurrence<NT, 64, XMAXR>(sR, sV, sDot, sNorm, mrows, lane, true);
+        } else if (mrows > XMAXR) {
+          if (lane == 0) sOverflow = 1;
+          __syncthreads();
+          failed = true;
+          break;
+        }
+        if (ok) done = true;
+      }
+      if (!done && !failed && sN <= nold) failed = true;
+    }
+    if (failed && lane == 0)
+      atomicOr(a.L.status, sOverflow ? ST_TOO_LARGE : ST_PATCH_FAIL);
+    if (done) {
+      __shared__ int64_t sOff;
+      __shared__ int32_t sItem;
+      const int ns = sN;
+      if (lane == 0) {
+        sOff = atomicAdd((unsigned long long*)a.L.x_cursor, (unsigned long long)ns);
+        sItem = (int32_t)atomicAdd((unsigned long long*)(a.L.x_cursor + 1), 1ull);
+      }
+      __syncthreads();
+      if (sOff + ns > a.L.x_cap || sItem >= a.L.cap) {
+        if (lane == 0) atomicOr(a.L.status, ST_XSET_OVERFLOW);
+      } else {
+        for (int i = lane; i < ns; i += 64) a.L.x_set[sOff + i] = sCur[i];
+        if (lane == 0) {
+          a.L.x_patch[sItem] = (int32_t)p;
+          a.L.x_off[sItem] = sOff;
+          a.L.x_len[sItem] = ns;
+          int b = bucket_of(ns * q);
+          int k = atomicAdd(a.L.count + NBUCKET + b, 1);
+          a.L.xitems[b][k] = sItem;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+inline unsigned grid_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+template <int NT, bool EXP>
+int fit_all(const FitArgs& a, hipStream_t s)
+{
+  // Persistent grids: each block loops over its share of the bucket's
+  // device-side item count.
+  const unsigned blocks = 8192;
+  k_exact_fit<NT, 16, 16, EXP><<<blocks, 64, 0, s>>>(a, 0);
+  k_exact_fit<NT, 32, 32, EXP><<<blocks, 64, 0, s>>>(a, 1);
+  k_exact_fit<NT, 64, 64, EXP><<<blocks, 64, 0, s>>>(a, 2);
+  k_exact_fit<NT, 96, 64, EXP><<<blocks, 64, 0, s>>>(a, 3);
+  k_exact_fit<NT, 128, 64, EXP><<<blocks, 64, 0, s>>>(a, 4);
+  k_exact_fit<NT, 192, 64, EXP><<<blocks / 4, 64, 0, s>>>(a, 5);
+  return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "exact fit launch failed");
+}
+
+}  // namespace
+
+int launch_classify(const FitArgs& a, int nterms, hipStream_t s)
+{
+  if (a.npatch == 0) return 0;
+  k_classify<<<grid_for(a.npatch, 256), 256, 0, s>>>(a, nterms);
+  return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "classify launch failed");
+}
+
+int launch_exact_fit(const FitArgs& a, int nterms, bool expanded, hipStream_t s)
+{
+  if (nterms == 4)
+    return expanded ? fit_all<4, true>(a, s) : fit_all<4, false>(a, s);
+  return expanded ? fit_all<10, true>(a, s) : fit_all<10, false>(a, s);
+}
+
+int launch_expand(const FitArgs& a, int nterms, hipStream_t s)
+{
+  const unsigned blocks = 4096;
+  if (nterms == 4)
+    k_expand<4><<<blocks, 64, 0, s>>>(a);
+  else
+    k_expand<10><<<blocks, 64, 0, s>>>(a);
+  return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "expand launch failed");
+}
+
+}  // namespace sprg

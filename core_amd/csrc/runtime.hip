@@ -1,0 +1,585 @@
+// runtime.hip -- the C ABI: contexts, device memory, the mesh mirror and the
+// SPR entry points (include/sprgpu.h).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "device.h"
+#include "spr_math.h"
+
+using namespace sprg;
+
+struct sprg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+namespace {
+
+// Grow-only device buffer.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n)
+  {
+    if (n <= bytes) return 0;
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, n) != hipSuccess) {
+      hipGetLastError();
+      return sprg_fail(SPRG_ENOMEM, "device allocation of %zu bytes failed", n);
+    }
+    bytes = n;
+    return 0;
+  }
+  void release()
+  {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct sprg_mesh {
+  sprg_ctx* ctx = nullptr;
+  int64_t nv = 0, nt = 0, ne = 0, nnodes = 0;
+  int order = 1;
+  int straight = 1;
+  DBuf tv, te, xyz, v2t_off, v2t, e2t_off, e2t;
+  // per-call workspace
+  DBuf lists, xset, rec, elem, partial, scalars, status, setlen;
+  int64_t list_cap = 0, xset_cap = 0;
+
+  MeshView view() const
+  {
+    MeshView m;
+    m.nv = nv; m.nt = nt; m.ne = ne;
+    m.order = order;
+    m.straight = straight;
+    m.tv = tv.as<int32_t>();
+    m.te = te.as<int32_t>();
+    m.xyz = xyz.as<double>();
+    m.v2t_off = v2t_off.as<int32_t>();
+    m.v2t = v2t.as<int32_t>();
+    m.e2t_off = e2t_off.as<int32_t>();
+    m.e2t = e2t.as<int32_t>();
+    return m;
+  }
+  int64_t npatch() const { return nv + (order == 2 ? ne : 0); }
+};
+
+#define HIP_OK(call, what)                                                    \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return sprg_fail(SPRG_EHIP, "%s: %s", what, hipGetErrorString(e_));     \
+  } while (0)
+
+extern "C" const char* sprg_version(void) { return "sprgpu 0.1 gfx950"; }
+
+extern "C" int sprg_ctx_create(int device, sprg_ctx** out)
+{
+  if (!out) return sprg_fail(SPRG_EINVAL, "ctx_create: null out");
+  *out = nullptr;
+  int n = 0;
+  HIP_OK(hipGetDeviceCount(&n), "hipGetDeviceCount");
+  if (device < 0 || device >= n)
+    return sprg_fail(SPRG_EINVAL, "ctx_create: device %d of %d", device, n);
+  HIP_OK(hipSetDevice(device), "hipSetDevice");
+  sprg_ctx* c = new (std::nothrow) sprg_ctx;
+  if (!c) return sprg_fail(SPRG_ENOMEM, "ctx_create: out of host memory");
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return sprg_fail(SPRG_EHIP, "ctx_create: stream creation failed");
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" int sprg_ctx_destroy(sprg_ctx* ctx)
+{
+  if (!ctx) return 0;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+extern "C" int sprg_ctx_sync(sprg_ctx* ctx)
+{
+  if (!ctx) return sprg_fail(SPRG_EINVAL, "ctx_sync: null ctx");
+  HIP_OK(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return 0;
+}
+
+extern "C" void* sprg_ctx_stream(sprg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+extern "C" int sprg_malloc(sprg_ctx* ctx, size_t bytes, void** dptr)
+{
+  if (!ctx || !dptr) return sprg_fail(SPRG_EINVAL, "malloc: null argument");
+  HIP_OK(hipSetDevice(ctx->device), "hipSetDevice");
+  if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess) {
+    hipGetLastError();
+    *dptr = nullptr;
+    return sprg_fail(SPRG_ENOMEM, "malloc: %zu bytes", bytes);
+  }
+  return 0;
+}
+
+extern "C" int sprg_free(sprg_ctx* ctx, void* dptr)
+{
+  if (!ctx) return sprg_fail(SPRG_EINVAL, "free: null ctx");
+  if (dptr) HIP_OK(hipFree(dptr), "hipFree");
+  return 0;
+}
+
+extern "C" int sprg_upload(sprg_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return sprg_fail(SPRG_EINVAL, "upload: null argument");
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "upload");
+  HIP_OK(hipStreamSynchronize(ctx->stream), "upload sync");
+  return 0;
+}
+
+extern "C" int sprg_download(sprg_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return sprg_fail(SPRG_EINVAL, "download: null argument");
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "download");
+  HIP_OK(hipStreamSynchronize(ctx->stream), "download sync");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// mesh mirror
+
+extern "C" int sprg_mesh_create(sprg_ctx* ctx, int64_t nv, int64_t nt, int64_t ne,
+                                const int32_t* tet_verts, const int32_t* tet_edges,
+                                const double* node_xyz, int order, sprg_mesh** out)
+{
+  if (!ctx || !out || !tet_verts || !node_xyz)
+    return sprg_fail(SPRG_EINVAL, "mesh_create: null argument");
+  *out = nullptr;
+  if (order != 1 && order != 2)
+    return sprg_fail(SPRG_EUNSUPPORTED, "mesh_create: coordinate order %d (1 or 2 supported)", order);
+  if (nv <= 0 || nt <= 0 || ne < 0)
+    return sprg_fail(SPRG_EINVAL, "mesh_create: empty mesh");
+  if (order == 2 && (!tet_edges || ne <= 0))
+    return sprg_fail(SPRG_EINVAL, "mesh_create: a Lagrange-2 mesh needs tet->edge");
+  if (4 * nt > INT32_MAX || 6 * nt > INT32_MAX || nv + ne > INT32_MAX)
+    return sprg_fail(SPRG_EUNSUPPORTED, "mesh_create: part too large for int32 indices");
+  // host-side validation: every index in range
+  for (int64_t i = 0; i < 4 * nt; ++i)
+    if (tet_verts[i] < 0 || tet_verts[i] >= nv)
+      return sprg_fail(SPRG_EINVAL, "mesh_create: tet %lld vertex index %d out of range",
+                       (long long)(i / 4), tet_verts[i]);
+  if (tet_edges)
+    for (int64_t i = 0; i < 6 * nt; ++i)
+      if (tet_edges[i] < 0 || tet_edges[i] >= ne)
+        return sprg_fail(SPRG_EINVAL, "mesh_create: tet %lld edge index %d out of range",
+                         (long long)(i / 6), tet_edges[i]);
+  HIP_OK(hipSetDevice(ctx->device), "hipSetDevice");
+  sprg_mesh* m = new (std::nothrow) sprg_mesh;
+  if (!m) return sprg_fail(SPRG_ENOMEM, "mesh_create: out of host memory");
+  m->ctx = ctx;
+  m->nv = nv;
+  m->nt = nt;
+  m->ne = tet_edges ? ne : 0;
+  m->order = order;
+  m->nnodes = nv + (order == 2 ? ne : 0);
+  // straight-sided test: every edge node is the midpoint (0 + a*0.5) + b*0.5
+  if (order == 2) {
+    for (int64_t t = 0; t < nt && m->straight; ++t)
+      for (int k = 0; k < 6 && m->straight; ++k) {
+        const double* a = node_xyz + 3 * (int64_t)tet_verts[4 * t + tet_edge_vert(k, 0)];
+        const double* b = node_xyz + 3 * (int64_t)tet_verts[4 * t + tet_edge_vert(k, 1)];
+        const double* c = node_xyz + 3 * (nv + (int64_t)tet_edges[6 * t + k]);
+        for (int j = 0; j < 3; ++j) {
+          double mid = 0;
+          mid += a[j] * 0.5;
+          mid += b[j] * 0.5;
+          if (mid != c[j]) m->straight = 0;
+        }
+      }
+  }
+  hipStream_t s = ctx->stream;
+  int rc = 0;
+  auto up = [&](DBuf& d, const void* src, size_t bytes) {
+    if (rc) return;
+    if ((rc = d.ensure(bytes))) return;
+    if (hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+      rc = sprg_fail(SPRG_EHIP, "mesh_create: upload failed");
+  };
+  up(m->tv, tet_verts, sizeof(int32_t) * 4 * nt);
+  if (tet_edges) up(m->te, tet_edges, sizeof(int32_t) * 6 * nt);
+  up(m->xyz, node_xyz, sizeof(double) * 3 * m->nnodes);
+  DBuf tmp;
+  int64_t tmpn = std::max(nv, m->ne) + 4;
+  if (!rc) rc = tmp.ensure(sizeof(int32_t) * tmpn);
+  if (!rc) rc = m->v2t_off.ensure(sizeof(int32_t) * (nv + 1));
+  if (!rc) rc = m->v2t.ensure(sizeof(int32_t) * 4 * nt);
+  if (!rc) rc = launch_csr(m->view(), 0, m->v2t_off.as<int32_t>(), m->v2t.as<int32_t>(), tmp.as<int32_t>(), s);
+  if (!rc && m->ne) {
+    rc = m->e2t_off.ensure(sizeof(int32_t) * (m->ne + 1));
+    if (!rc) rc = m->e2t.ensure(sizeof(int32_t) * 6 * nt);
+    if (!rc) rc = launch_csr(m->view(), 1, m->e2t_off.as<int32_t>(), m->e2t.as<int32_t>(), tmp.as<int32_t>(), s);
+  }
+  if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = sprg_fail(SPRG_EHIP, "mesh_create: sync failed");
+  tmp.release();
+  if (rc) {
+    sprg_mesh_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return 0;
+}
+
+extern "C" int sprg_mesh_destroy(sprg_mesh* m)
+{
+  if (!m) return 0;
+  hipSetDevice(m->ctx->device);
+  hipStreamSynchronize(m->ctx->stream);
+  DBuf* all[] = {&m->tv, &m->te, &m->xyz, &m->v2t_off, &m->v2t, &m->e2t_off, &m->e2t,
+                 &m->lists, &m->xset, &m->rec, &m->elem, &m->partial, &m->scalars,
+                 &m->status, &m->setlen};
+  for (DBuf* b : all) b->release();
+  delete m;
+  return 0;
+}
+
+extern "C" int sprg_mesh_counts(sprg_mesh* m, int64_t* counts)
+{
+  if (!m || !counts) return sprg_fail(SPRG_EINVAL, "mesh_counts: null argument");
+  counts[0] = m->nv;
+  counts[1] = m->nt;
+  counts[2] = m->ne;
+  counts[3] = m->nnodes;
+  return 0;
+}
+
+extern "C" int sprg_mesh_adjacency(sprg_mesh* m, int dim, const int32_t** off, const int32_t** idx)
+{
+  if (!m || !off || !idx) return sprg_fail(SPRG_EINVAL, "mesh_adjacency: null argument");
+  if (dim == 0) {
+    *off = m->v2t_off.as<int32_t>();
+    *idx = m->v2t.as<int32_t>();
+    return 0;
+  }
+  if (dim == 1 && m->ne) {
+    *off = m->e2t_off.as<int32_t>();
+    *idx = m->e2t.as<int32_t>();
+    return 0;
+  }
+  return sprg_fail(SPRG_EINVAL, "mesh_adjacency: dimension %d not available", dim);
+}
+
+// ---------------------------------------------------------------------------
+// recovery pipeline
+
+namespace {
+
+int check_ip(sprg_mesh* m, int ip_order, int nc, int* q)
+{
+  *q = rule_npoints(ip_order);
+  if (*q < 0) return sprg_fail(SPRG_EUNSUPPORTED, "integration order %d (1..3 supported)", ip_order);
+  // getSamplePoints indexes the mesh-order rule by IP index; the order-2
+  // rule has 4 points (the N1 rule of an order-1 mesh ignores the index).
+  if (m->order == 2 && *q > 4)
+    return sprg_fail(SPRG_EUNSUPPORTED, "%d points per element exceed the order-2 mesh rule", *q);
+  if (nc < 0 || nc > 27) return sprg_fail(SPRG_EINVAL, "component count %d", nc);
+  return 0;
+}
+
+// Lays out the work lists in one device allocation.
+int setup_lists(sprg_mesh* m, FitArgs& a)
+{
+  const int64_t np = m->npatch();
+  if (m->list_cap < np) {
+    // items and xitems per bucket, xq, x_patch, x_len: int32; x_off: int64;
+    // counts, cursors, status
+    size_t n32 = (size_t)np * (2 * NBUCKET + 3) + 2 * NBUCKET + 8;
+    size_t bytes = n32 * 4 + (size_t)np * 8 + 64;
+    int rc = m->lists.ensure(bytes);
+    if (rc) return rc;
+    m->list_cap = np;
+  }
+  if (m->xset_cap == 0) {
+    int64_t cap = 8 * np + (1 << 20);
+    int rc = m->xset.ensure(sizeof(int32_t) * cap);
+    if (rc) return rc;
+    m->xset_cap = cap;
+  }
+  char* base = m->lists.as<char>();
+  int64_t* x_off = (int64_t*)base;  // 8-byte aligned first
+  int64_t* x_cursor = x_off + m->list_cap;
+  int32_t* p = (int32_t*)(x_cursor + 2);
+  Lists& L = a.L;
+  L.x_off = x_off;
+  L.x_cursor = x_cursor;
+  L.count = p;
+  p += 2 * NBUCKET + 2;
+  L.status = p;
+  p += 2;
+  for (int b = 0; b < NBUCKET; ++b) { L.items[b] = p; p += m->list_cap; }
+  for (int b = 0; b < NBUCKET; ++b) { L.xitems[b] = p; p += m->list_cap; }
+  L.xq = p; p += m->list_cap;
+  L.x_patch = p; p += m->list_cap;
+  L.x_len = p; p += m->list_cap;
+  L.x_set = m->xset.as<int32_t>();
+  L.x_cap = m->xset_cap;
+  L.cap = m->list_cap;
+  return 0;
+}
+
+int read_status(sprg_mesh* m, const FitArgs& a, int32_t* st)
+{
+  hipStream_t s = m->ctx->stream;
+  HIP_OK(hipMemcpyAsync(st, a.L.status, sizeof(int32_t), hipMemcpyDeviceToHost, s), "status");
+  HIP_OK(hipStreamSynchronize(s), "recover sync");
+  return 0;
+}
+
+// classify -> exact fit of the initial sets -> expansion -> exact fit of the
+// expanded sets.  Retries once with larger expanded-set storage.
+int run_recover(sprg_mesh* m, int q, int nc, const double* ip, double* out, int form)
+{
+  if (form != SPRG_FP64)
+    return sprg_fail(SPRG_EUNSUPPORTED, "form %d is not available in this build", form);
+  const int nterms = m->order == 1 ? 4 : 10;
+  hipStream_t s = m->ctx->stream;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    FitArgs a;
+    a.m = m->view();
+    a.q = q;
+    a.nc = nc;
+    a.npatch = m->npatch();
+    a.ip = ip;
+    a.out = out;
+    int rc = setup_lists(m, a);
+    if (rc) return rc;
+    HIP_OK(hipMemsetAsync(a.L.x_cursor, 0, sizeof(int64_t) * 2, s), "memset");
+    HIP_OK(hipMemsetAsync(a.L.count, 0, sizeof(int32_t) * (2 * NBUCKET + 4), s), "memset");
+    if ((rc = launch_classify(a, nterms, s))) return rc;
+    if ((rc = launch_exact_fit(a, nterms, false, s))) return rc;
+    if ((rc = launch_expand(a, nterms, s))) return rc;
+    if ((rc = launch_exact_fit(a, nterms, true, s))) return rc;
+    int32_t st = 0;
+    if ((rc = read_status(m, a, &st))) return rc;
+    if (st & ST_XSET_OVERFLOW) {
+      int64_t cap = 2 * m->xset_cap;
+      if ((rc = m->xset.ensure(sizeof(int32_t) * cap))) return rc;
+      m->xset_cap = cap;
+      continue;
+    }
+    if (st & ST_PATCH_FAIL) return sprg_fail(SPRG_EPATCH, "SPR: patch construction: all hope is lost.");
+    if (st & ST_TOO_LARGE)
+      return sprg_fail(SPRG_EUNSUPPORTED, "SPR: a patch needs more than %d sample rows", bucket_rows(NBUCKET - 1));
+    if (st & ST_INTERNAL) return sprg_fail(SPRG_EINTERNAL, "SPR: inconsistent rank decision");
+    return 0;
+  }
+  return sprg_fail(SPRG_ENOMEM, "SPR: expanded patch storage kept overflowing");
+}
+
+}  // namespace
+
+extern "C" int sprg_grad_ip(sprg_mesh* m, int field_order, int nc_in, const double* nodal,
+                            int ip_order, double* ip)
+{
+  if (!m || !nodal || !ip) return sprg_fail(SPRG_EINVAL, "grad_ip: null argument");
+  if (nc_in != 1 && nc_in != 3 && nc_in != 9)
+    return sprg_fail(SPRG_EINVAL, "getGradIPField: value type must be scalar, vector or matrix");
+  if (field_order != 1 && field_order != 2)
+    return sprg_fail(SPRG_EUNSUPPORTED, "getGradIPField: field order %d (1 or 2 supported)", field_order);
+  if (field_order == 2 && !m->ne)
+    return sprg_fail(SPRG_EINVAL, "getGradIPField: an order-2 field needs mesh edges");
+  if (rule_npoints(ip_order) < 0)
+    return sprg_fail(SPRG_EUNSUPPORTED, "getGradIPField: integration order %d (1..3 supported)", ip_order);
+  HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  int rc = launch_grad(m->view(), field_order, nc_in, nodal, ip_order, ip, m->ctx->stream);
+  if (rc) return rc;
+  HIP_OK(hipStreamSynchronize(m->ctx->stream), "grad sync");
+  return 0;
+}
+
+extern "C" int sprg_recover(sprg_mesh* m, int ip_order, int nc, const double* ip, double* nodal,
+                            int form)
+{
+  if (!m || !ip || !nodal) return sprg_fail(SPRG_EINVAL, "recover: null argument");
+  int q, rc = check_ip(m, ip_order, nc, &q);
+  if (rc) return rc;
+  HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  return run_recover(m, q, nc, ip, nodal, form);
+}
+
+namespace {
+
+int run_estimate(sprg_mesh* m, int ip_order, int nc, const double* ip, int target,
+                 double ar, double tn, double alpha, double beta, double* size, int form,
+                 double* diag)
+{
+  auto t0 = std::chrono::steady_clock::now();
+  int q, rc = check_ip(m, ip_order, nc, &q);
+  if (rc) return rc;
+  if (nc != 1 && nc != 3 && nc != 9 && nc != 27)
+    return sprg_fail(SPRG_EUNSUPPORTED, "estimator: %d components", nc);
+  HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  if ((rc = m->rec.ensure(sizeof(double) * m->nnodes * nc))) return rc;
+  if ((rc = run_recover(m, q, nc, ip, m->rec.as<double>(), form))) return rc;
+  int64_t nblocks = (m->nt + 255) / 256;
+  if ((rc = m->elem.ensure(sizeof(double) * 2 * m->nt))) return rc;
+  if ((rc = m->partial.ensure(sizeof(double) * 2 * nblocks))) return rc;
+  if ((rc = m->scalars.ensure(sizeof(double) * 4))) return rc;
+  EstArgs e;
+  e.m = m->view();
+  e.ip_order = ip_order;
+  e.nc = nc;
+  e.ip = ip;
+  e.rec = m->rec.as<double>();
+  e.target = target;
+  e.adapt_ratio = ar;
+  e.target_n = tn;
+  e.alpha = alpha;
+  e.beta = beta;
+  e.elem = m->elem.as<double>();
+  e.partial = m->partial.as<double>();
+  e.scalars = m->scalars.as<double>();
+  e.size = size;
+  e.nblocks = (int)nblocks;
+  if ((rc = launch_estimate(e, m->ctx->stream))) return rc;
+  double sc[4] = {0, 0, 0, 0};
+  HIP_OK(hipMemcpyAsync(sc, e.scalars, sizeof(double) * 3, hipMemcpyDeviceToHost, m->ctx->stream),
+         "scalars");
+  HIP_OK(hipStreamSynchronize(m->ctx->stream), "estimate sync");
+  auto t1 = std::chrono::steady_clock::now();
+  if (diag) {
+    if (!target) {
+      diag[0] = sc[0];
+      diag[1] = sc[1];
+      diag[2] = sc[2];
+    } else {
+      diag[0] = sc[1];
+      diag[1] = sc[2];
+      diag[2] = 0;
+    }
+    diag[3] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int sprg_spr_size(sprg_mesh* m, int ip_order, int nc, const double* ip,
+                             double adapt_ratio, double* size, int form, double* diag)
+{
+  if (!m || !ip || !size) return sprg_fail(SPRG_EINVAL, "spr_size: null argument");
+  return run_estimate(m, ip_order, nc, ip, 0, adapt_ratio, 0, 0, 0, size, form, diag);
+}
+
+extern "C" int sprg_target_spr_size(sprg_mesh* m, int ip_order, int nc, const double* ip,
+                                    double target, double alpha, double beta, double* size,
+                                    int form, double* diag)
+{
+  if (!m || !ip || !size) return sprg_fail(SPRG_EINVAL, "target_spr_size: null argument");
+  // PCU_ALWAYS_ASSERT(target > 0), PCU_ALWAYS_ASSERT(alpha < beta)
+  // (spr/sprEstimateTargetError.cc:249-250)
+  if (!(target > 0)) return sprg_fail(SPRG_EINVAL, "getTargetSPRSizeField: target must be > 0");
+  if (!(alpha < beta)) return sprg_fail(SPRG_EINVAL, "getTargetSPRSizeField: alpha must be < beta");
+  return run_estimate(m, ip_order, nc, ip, 1, 0, target, alpha, beta, size, form, diag);
+}
+
+// ---------------------------------------------------------------------------
+// patch sets (for parity checks of the patch construction)
+
+namespace {
+
+__global__ void k_set_len(FitArgs a, int32_t* len, int32_t* xmap)
+{
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= a.npatch) return;
+  int32_t k = xmap[p];
+  if (k >= 0) {
+    len[p] = a.L.x_len[k];
+  } else if (p < a.m.nv) {
+    len[p] = a.m.v2t_off[p + 1] - a.m.v2t_off[p];
+  } else {
+    len[p] = a.m.e2t_off[p - a.m.nv + 1] - a.m.e2t_off[p - a.m.nv];
+  }
+}
+
+__global__ void k_xmap(FitArgs a, int32_t* xmap, int64_t nitems)
+{
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nitems) return;
+  xmap[a.L.x_patch[k]] = (int32_t)k;
+}
+
+__global__ void k_set_fill(FitArgs a, const int32_t* incl, const int32_t* xmap,
+                           int64_t* set_off, int32_t* set_idx, int64_t cap)
+{
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= a.npatch) return;
+  int64_t beg = p ? incl[p - 1] : 0, end = incl[p];
+  set_off[p] = beg;
+  if (p == a.npatch - 1) set_off[p + 1] = end;
+  if (!set_idx || end > cap) return;
+  const int32_t* src;
+  int32_t k = xmap[p];
+  if (k >= 0)
+    src = a.L.x_set + a.L.x_off[k];
+  else if (p < a.m.nv)
+    src = a.m.v2t + a.m.v2t_off[p];
+  else
+    src = a.m.e2t + a.m.e2t_off[p - a.m.nv];
+  for (int64_t i = 0; i < end - beg; ++i) set_idx[beg + i] = src[i];
+}
+
+}  // namespace
+
+extern "C" int sprg_patch_sets(sprg_mesh* m, int ip_order, int64_t* set_off, int32_t* set_idx,
+                               int64_t cap, int64_t* total)
+{
+  if (!m || !set_off || !total) return sprg_fail(SPRG_EINVAL, "patch_sets: null argument");
+  int q, rc = check_ip(m, ip_order, 0, &q);
+  if (rc) return rc;
+  HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  // the fit with zero components runs the rank tests and the expansion only
+  double dummy_out = 0;
+  (void)dummy_out;
+  if ((rc = m->rec.ensure(sizeof(double)))) return rc;
+  if ((rc = run_recover(m, q, 0, m->rec.as<double>(), m->rec.as<double>(), SPRG_FP64))) return rc;
+  FitArgs a;
+  a.m = m->view();
+  a.q = q;
+  a.nc = 0;
+  a.npatch = m->npatch();
+  if ((rc = setup_lists(m, a))) return rc;
+  hipStream_t s = m->ctx->stream;
+  int64_t nitems = 0;
+  HIP_OK(hipMemcpyAsync(&nitems, a.L.x_cursor + 1, sizeof(int64_t), hipMemcpyDeviceToHost, s), "items");
+  HIP_OK(hipStreamSynchronize(s), "sync");
+  const int64_t np = a.npatch;
+  if ((rc = m->setlen.ensure(sizeof(int32_t) * (3 * np + 1024)))) return rc;
+  int32_t* len = m->setlen.as<int32_t>();
+  int32_t* xmap = len + np;
+  int32_t* work = xmap + np;
+  HIP_OK(hipMemsetAsync(xmap, 0xff, sizeof(int32_t) * np, s), "memset");
+  unsigned g = (unsigned)((np + 255) / 256);
+  if (nitems) k_xmap<<<(unsigned)((nitems + 255) / 256), 256, 0, s>>>(a, xmap, nitems);
+  k_set_len<<<g, 256, 0, s>>>(a, len, xmap);
+  if ((rc = launch_scan_i32(len, np, work, s))) return rc;
+  int32_t tot = 0;
+  HIP_OK(hipMemcpyAsync(&tot, len + np - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s), "total");
+  HIP_OK(hipStreamSynchronize(s), "sync");
+  *total = tot;
+  k_set_fill<<<g, 256, 0, s>>>(a, len, xmap, set_off, set_idx, cap);
+  HIP_OK(hipStreamSynchronize(s), "sync");
+  return 0;
+}

@@ -1,0 +1,139 @@
+/*
+ * sprgpu.h -- C ABI of the MI355X (gfx950) Superconvergent Patch Recovery
+ * library.  It replaces the error-estimation path of SCOREC/core's `spr`
+ * library (spr/spr.h:33-63) for one mesh part of tetrahedra; the apf
+ * adapter that maps apf::Field* in/out onto these calls is given in
+ * INTEGRATION.md.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or an SPRG_E* code; the message
+ *     of the calling thread's last failure is sprg_last_error().  Nothing
+ *     aborts: the reference's PCU_ALWAYS_ASSERT / apf::fail conditions come
+ *     back as codes (SPRG_EPATCH is "SPR: patch construction: all hope is
+ *     lost.", spr/sprRecoverField.cc:383).
+ *   - Field arrays passed to compute calls are DEVICE pointers (allocate
+ *     them with hipMalloc, torch, or sprg_malloc).  Mesh arrays passed to
+ *     sprg_mesh_create are HOST pointers; the library keeps its own device
+ *     mirror.
+ *   - Layouts are the apf ones:
+ *       nodal field  [N][nc], N = nv (order 1) or nv + ne (order 2):
+ *                    vertex nodes first, then one node per edge
+ *       IP field     [nt][q][nc], q = points of the tet rule of the IP
+ *                    order (1 -> 1, 2 -> 4, 3 -> 5), point order of
+ *                    apf/apfIntegrate.cc:316-361
+ *       value types  nc = 1 scalar, 3 vector, 9 matrix (row-major 3x3)
+ *   - A context owns one HIP stream; calls on one context are serialised
+ *     and are not re-entrant; distinct contexts are independent.  Every
+ *     compute call is stream-ordered; sprg_ctx_sync waits for it.
+ */
+#ifndef SPRGPU_H
+#define SPRGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SPRG_OK = 0,
+  SPRG_EINVAL = 1,       /* bad argument (PCU_ALWAYS_ASSERT in the reference) */
+  SPRG_ENOMEM = 2,       /* device allocation failed */
+  SPRG_EHIP = 3,         /* HIP runtime error */
+  SPRG_EPATCH = 4,       /* patch cannot reach full rank */
+  SPRG_EUNSUPPORTED = 5, /* input the reference handles that this build does not */
+  SPRG_EINTERNAL = 6
+};
+
+/* Numeric form of the recovery and estimation kernels. */
+enum {
+  SPRG_FP64 = 0,  /* plain FP64 form */
+  SPRG_FP32R = 1  /* FP32 patch solves + one FP64 residual refinement step */
+};
+
+typedef struct sprg_ctx sprg_ctx;
+typedef struct sprg_mesh sprg_mesh;
+
+const char* sprg_last_error(void);
+/* Library version, e.g. "sprgpu 0.1 gfx950". */
+const char* sprg_version(void);
+
+/* ---- context and device memory --------------------------------------- */
+int sprg_ctx_create(int device, sprg_ctx** out);
+int sprg_ctx_destroy(sprg_ctx* ctx);
+int sprg_ctx_sync(sprg_ctx* ctx);
+/* The context's HIP stream (hipStream_t), for callers that order their own
+ * work against it. */
+void* sprg_ctx_stream(sprg_ctx* ctx);
+int sprg_malloc(sprg_ctx* ctx, size_t bytes, void** dptr);
+int sprg_free(sprg_ctx* ctx, void* dptr);
+int sprg_upload(sprg_ctx* ctx, void* dst, const void* src, size_t bytes);
+int sprg_download(sprg_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- mesh mirror ------------------------------------------------------
+ * Replaces the apf::Mesh2 queries the reference path makes on one part:
+ * m->begin(3) element order, getDownward(tet, 0|1) (tet_verts [nt][4],
+ * tet_edges [nt][6] in apf::tet_edge_verts order), the coordinate field
+ * (node_xyz [N][3], vertices then edge nodes for a Lagrange-2 mesh) and
+ * getAdjacent(vertex|edge, 3) (built on the device as CSR sorted by tet
+ * index, mds/mds.c:726-743 with the handle order of mds/apfMDS.cc:98-107).
+ * tet_edges may be NULL for order 1; ne is then 0. */
+int sprg_mesh_create(sprg_ctx* ctx, int64_t nv, int64_t nt, int64_t ne,
+                     const int32_t* tet_verts, const int32_t* tet_edges,
+                     const double* node_xyz, int order, sprg_mesh** out);
+int sprg_mesh_destroy(sprg_mesh* mesh);
+/* counts[0..3] = nv, nt, ne, nodes */
+int sprg_mesh_counts(sprg_mesh* mesh, int64_t* counts);
+/* Device CSR of getAdjacent(entity, 3): dim 0 (vertices) or 1 (edges).
+ * off has n+1 entries, idx has off[n]; both stay owned by the mesh. */
+int sprg_mesh_adjacency(sprg_mesh* mesh, int dim, const int32_t** off,
+                        const int32_t** idx);
+
+/* ---- spr::getGradIPField (spr/sprGetGradIPField.cc:15-65) -------------
+ * nodal [N][nc_in] of a Lagrange field of field_order (1|2) -> ip
+ * [nt][q][3*nc_in]; nc_in 1 -> 3 (gradient), 3 -> 9 (g[i][j] = du_j/dx_i),
+ * 9 -> 27 (apf PACKED layout i*3+j+9*d). */
+int sprg_grad_ip(sprg_mesh* mesh, int field_order, int nc_in,
+                 const double* nodal, int ip_order, double* ip);
+
+/* ---- patch sets of spr::recoverField (spr/sprRecoverField.cc:155-393) --
+ * For every recovered entity (vertices, then edges of an order-2 mesh) the
+ * tet set its polynomial fit uses, after expansion.  Writes
+ * set_off[npatch+1] (device, int64) and, if set_idx is not NULL and cap is
+ * large enough, set_idx[set_off[npatch]] (device, int32).  *total gets the
+ * required size.  Rank decisions follow mth/mthQR.cc:31 (absolute column
+ * norm threshold 1e-10 on raw-coordinate monomials). */
+int sprg_patch_sets(sprg_mesh* mesh, int ip_order, int64_t* set_off,
+                    int32_t* set_idx, int64_t cap, int64_t* total);
+
+/* ---- spr::recoverField (spr/sprRecoverField.cc:419-428) ----------------
+ * ip [nt][q][nc] -> nodal [N][nc], N = nodes of the mesh's coordinate
+ * order (the reference's "spr_<name>" Lagrange field). */
+int sprg_recover(sprg_mesh* mesh, int ip_order, int nc, const double* ip,
+                 double* nodal, int form);
+
+/* ---- spr::getSPRSizeField (spr/sprEstimateError.cc:378-388) ------------
+ * ip [nt][q][nc] -> size [nv].  diag (host, optional, 4 doubles):
+ * ||eps*||^2, sum of element error terms, size_factor, wall ms. */
+int sprg_spr_size(sprg_mesh* mesh, int ip_order, int nc, const double* ip,
+                  double adapt_ratio, double* size, int form, double* diag);
+
+/* ---- spr::getTargetSPRSizeField (spr/sprEstimateTargetError.cc:241-258)
+ * diag (host, optional, 4 doubles): G, size_factor, 0, wall ms. */
+int sprg_target_spr_size(sprg_mesh* mesh, int ip_order, int nc,
+                         const double* ip, double target, double alpha,
+                         double beta, double* size, int form, double* diag);
+
+/* ---- synthetic box meshes (host; mds/apfBox.cc:178-336 numbering) ------ */
+int sprg_box_counts(int n, int64_t* nv, int64_t* nt, int64_t* ne);
+/* Fills tv[nt][4], te[nt][6] (may be NULL for order 1), ev[ne][2] (may be
+ * NULL) and xyz[nv (+ ne for order 2)][3].  jitter is the interior-vertex
+ * displacement bound in lattice units (0 = none), grade != 0 maps x -> x^2. */
+int sprg_box_build(int n, int order, double jitter, uint64_t seed, int grade,
+                   int32_t* tv, int32_t* te, int32_t* ev, double* xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,138 @@
+"""GPU parity of the FP64 form against the FP64 oracle (small meshes).
+
+Bit-exact: upward adjacency, patch sets, integration-point gradients and
+recovered fields.  The size fields differ from the oracle only through the
+summation order of two global sums and of the vertex average, so they are
+held to 1e-13 relative (the north-star bar for the FP64 form is 1e-12).
+"""
+import numpy as np
+import pytest
+
+from core_amd import _lib, mesh, spr, workloads
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# name, n, order, jitter, seed, grade
+CASES = [
+    ("p1", 4, 1, 0.0, 0, False),
+    ("p1_jitter", 5, 1, 0.25, 11, False),
+    ("p2", 3, 2, 0.0, 0, False),
+    ("p2_jitter", 4, 2, 0.25, workloads.C5, False),
+    ("p2_graded", 4, 2, 0.3, workloads.C3, True),
+]
+
+
+def _host(case):
+    _, n, order, jit, seed, grade = case
+    return mesh.box(n, order=order, jitter=jit, seed=seed, grade=grade)
+
+
+def _omesh(h):
+    return O.Mesh(h.tv, h.xyz, h.order, te=h.te, ev=h.ev)
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def case(request, gpu):
+    h = _host(request.param)
+    return h, spr.Mesh(h), _omesh(h)
+
+
+def test_adjacency_bitwise(case):
+    h, m, om = case
+    off, idx = m.adjacency(0)
+    ooff, oidx = O.vertex_tets(om)
+    assert np.array_equal(off, ooff) and np.array_equal(idx, oidx)
+    if h.order == 2:
+        off, idx = m.adjacency(1)
+        ooff, oidx = O.edge_tets(om)
+        assert np.array_equal(off, ooff) and np.array_equal(idx, oidx)
+
+
+@pytest.mark.parametrize("ip_order", [1, 2])
+def test_patch_sets_bitwise(case, ip_order):
+    h, m, om = case
+    q = {1: 1, 2: 4}[ip_order]
+    off, idx = m.patch_sets(ip_order)
+    ooff, oidx = O.patch_sets(om, q)
+    assert np.array_equal(off, ooff.astype(np.int64))
+    assert np.array_equal(idx, oidx)
+
+
+@pytest.mark.parametrize("ncomp", [1, 3, 9])
+@pytest.mark.parametrize("ip_order", [1, 2])
+def test_grad_ip_bitwise(case, ncomp, ip_order):
+    h, m, om = case
+    P = workloads.node_points(h)
+    rng = np.random.default_rng(ncomp * 10 + ip_order)
+    base = np.stack([workloads.sin_field(P), workloads.exp_field(P), P[:, 0] * P[:, 1]], 1)
+    vals = base[:, :1] if ncomp == 1 else np.concatenate([base] * (ncomp // 3), 1)
+    vals = vals + 0.1 * rng.standard_normal(vals.shape)
+    f = m.createLagrangeField("u%d_%d" % (ncomp, ip_order), ncomp, h.order, vals)
+    g = spr.getGradIPField(f, "g%d_%d" % (ncomp, ip_order), ip_order)
+    ref = O.grad_ip(om, h.order, vals, ip_order)
+    got = g.numpy()
+    m.destroyField(f)
+    m.destroyField(g)
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("ip_order", [1, 2])
+def test_recover_bitwise(case, ip_order):
+    h, m, om = case
+    ip = workloads.strain_field(h.ip_points(ip_order))
+    f = m.createIPField("eps_r%d" % ip_order, 9, ip_order, ip)
+    r = spr.recoverField(f)
+    got = r.numpy()
+    m.destroyField(f)
+    m.destroyField(r)
+    ref = O.recover(om, ip)
+    assert np.array_equal(got, ref)
+
+
+def test_spr_size(case):
+    h, m, om = case
+    ip = workloads.strain_field(h.ip_points(2))
+    f = m.createIPField("eps_s", 9, 2, ip)
+    diag = {}
+    s = spr.getSPRSizeField(f, 0.05, diag=diag)
+    got = s.numpy()[:, 0]
+    m.destroyField(f)
+    m.destroyField(s)
+    ref, odiag = O.spr_size(om, 2, ip, 0.05)
+    assert np.max(np.abs(got / ref - 1)) <= 1e-13
+    assert abs(diag["size_factor"] / odiag[2] - 1) <= 1e-13
+
+
+def test_target_size(case):
+    h, m, om = case
+    P = workloads.node_points(h)
+    u = m.createLagrangeField("u_t", 1, h.order, workloads.exp_field(P))
+    g = spr.getGradIPField(u, "g_t", 2)
+    ipv = g.numpy()
+    s = spr.getTargetSPRSizeField(g, 2 * h.nt, 0.25, 2.0)
+    got = s.numpy()[:, 0]
+    for f in (u, g, s):
+        m.destroyField(f)
+    ref, _ = O.target_size(om, 2, ipv, 2 * h.nt, 0.25, 2.0)
+    assert np.max(np.abs(got / ref - 1)) <= 1e-13
+
+
+def test_patch_failure_is_reported(gpu):
+    # one P2 tet: a vertex patch has 4 samples for 10 terms and nothing to
+    # grow into -> "SPR: patch construction: all hope is lost."
+    tv = np.array([[0, 1, 2, 3]], np.int32)
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    te = np.arange(6, dtype=np.int32).reshape(1, 6)
+    ev = mesh.TET_EDGE_VERTS.astype(np.int32)
+    mid = 0.5 * X[ev[:, 0]] + 0.5 * X[ev[:, 1]]
+    h = mesh.HostMesh(tv, np.vstack([X, mid]), 2, te=te, ev=ev, ne=6)
+    m = spr.Mesh(h)
+    f = m.createIPField("eps", 3, 2, np.ones((1, 4, 3)))
+    with pytest.raises(_lib.SPRError) as e:
+        spr.recoverField(f)
+    assert e.value.code == _lib.SPRG_EPATCH
+    om = _omesh(h)
+    with pytest.raises(RuntimeError):
+        O.recover(om, np.ones((1, 4, 3)))
