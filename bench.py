@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""SPR size-field benchmark on one MI355X.
+
+Workload (strain_120): the unit cube as 120^3 hexes x 6 Kuhn tets
+(10,368,000 tets, 1,771,561 vertices, 12,225,960 edges), Lagrange-2
+coordinates, interior jitter 0.25h (seed 0x5EED0005); a 9-component strain
+IP field eps = sym(grad v), v = (sin(pi x) y, z^2, cos(pi y) x), at the 4
+points of the order-2 tet rule; one step = spr::getSPRSizeField(eps, 0.05)
+(patch recovery of all vertex and edge patches, element error integration,
+size factor, vertex average) with the IP field resident in HBM.
+
+Prints one JSON line.  value = recovered patches (V + E) per second of the
+whole size-field step; roofline = the recovery phase's algorithmic HBM bytes
+over its GPU time (HIP events inside the library on its stream);
+cpu_baseline = the FP64 CPU oracle on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(T, V, E, q=4, c=9, order=2):
+    """Compulsory HBM bytes per phase (FP64 data, int32 indices)."""
+    N = V + (E if order == 2 else 0)
+    k = 10 if order == 2 else 4
+    up = 4 * T + (6 * T if order == 2 else 0)
+    recover = T * q * c * 8 + N * 24 + T * k * 4 + up * 4 + (N + 1) * 4 + N * c * 8
+    size = T * q * c * 8 + N * c * 8 + T * k * 4 + N * 24 + T * 8
+    average = T * 8 + 4 * T * 4 + (V + 1) * 4 + V * 8
+    return {"recover": recover, "size": size, "average": average}
+
+
+def cpu_baseline(host, ip, budget_s):
+    """Time the FP64 oracle on a sample of patches and tets of the workload
+    and scale to the whole step (serial, one core)."""
+    import numpy as np
+    from oracle import oracle as O
+
+    om = O.Mesh(host.tv, host.xyz, host.order, te=host.te, ev=host.ev)
+    V, E, T = host.nv, host.ne, host.nt
+    rec = np.zeros((host.nnodes, ip.shape[2]))
+
+    def timed(fn, *a):
+        t = time.perf_counter()
+        fn(*a)
+        return time.perf_counter() - t
+
+    # every oracle recover call rebuilds the mesh's upward adjacency first;
+    # that fixed cost (an empty range) is mesh setup, not per-patch work
+    fixed = min(timed(O.recover, om, ip, 0, 0) for _ in range(2))
+    mid_v, mid_e, mid_t = V // 2, V + E // 2, T // 2
+
+    def costs(nv_s, ne_s, nt_s):
+        tv = max(timed(O.recover, om, ip, mid_v, mid_v + nv_s) - fixed, 1e-9) / nv_s
+        te = max(timed(O.recover, om, ip, mid_e, mid_e + ne_s) - fixed, 1e-9) / ne_s
+        tt = timed(O.element_error_range, om, 2, ip, rec, mid_t, mid_t + nt_s) / nt_s
+        return tv, te, tt
+
+    tv, te, tt = costs(50, 300, 5000)  # probe, then size the sample to the budget
+    frac = budget_s / (tv * V + te * E + tt * T)
+    nv_s, ne_s, nt_s = max(50, int(V * frac)), max(300, int(E * frac)), max(5000, int(T * frac))
+    tv, te, tt = costs(nv_s, ne_s, nt_s)
+    step_s = tv * V + te * E + tt * T
+    return {
+        "value": (V + E) / step_s,
+        "unit": "patches/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": ("FP64 oracle (reference operation order, explicit-Q Householder), serial: "
+                   "%d vertex patches + %d edge patches recovered and %d tets integrated from the "
+                   "middle of the mesh, per-item times scaled to V+E patches and T tets "
+                   "(est. full CPU step %.0f s; vertex patch %.3f ms, edge patch %.3f ms, "
+                   "tet %.2f us)" % (nv_s, ne_s, nt_s, step_s, tv * 1e3, te * 1e3, tt * 1e6)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--form", default="fp32r", choices=["fp64", "fp32r", "fp64f"])
+    ap.add_argument("--n", type=int, default=120)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from core_amd import spr, workloads
+
+    t0 = time.time()
+    host = workloads.strain_mesh(args.n)
+    log("mesh n=%d: %d tets, %d vertices, %d edges (%.1fs)" % (args.n, host.nt, host.nv, host.ne,
+                                                                time.time() - t0))
+    ip = workloads.strain_ip(host, progress=lambda d, n: log("  strain IP field %d/%d tets" % (d, n))
+                             if d % (1 << 23) < (1 << 20) or d == n else None)
+    log("strain IP field %s (%.1fs)" % (ip.shape, time.time() - t0))
+    m = spr.Mesh(host)
+    eps = m.createIPField("eps", 9, 2, ip)
+    torch.cuda.synchronize()
+    log("uploaded (%.1fs)" % (time.time() - t0))
+
+    def step():
+        s = spr.getSPRSizeField(eps, 0.05, form=args.form)
+        m.destroyField(s)
+        return m.phase_times()
+
+    for i in range(args.warmup):
+        step()
+        log("warmup %d done" % i)
+    phases = np.zeros(5)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        phases += step()
+    torch.cuda.synchronize()
+    t_end = time.perf_counter()
+    ms_per_step = (t_end - t_start) * 1e3 / args.steps
+    phases /= args.steps
+    log("timed %d steps: %.3f ms/step; phases ms %s" % (args.steps, ms_per_step,
+                                                       np.array2string(phases, precision=3)))
+    T, V, E = host.nt, host.nv, host.ne
+    patches = V + E
+    value = patches / (ms_per_step / 1e3)
+    ab = algorithmic_bytes(T, V, E)
+    rec_ms = float(phases[0] + phases[1] + phases[2])
+    achieved = ab["recover"] / (rec_ms / 1e3) / 1e9
+    out = {
+        "metric": "SPR patches/s & size-field ms @10.4M tets: FP32+refine vs FP64, rel err<=1e-6",
+        "value": value,
+        "unit": "patches/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.form == "fp32r" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "strain_120: getSPRSizeField(0.05) on n=120 P2 jittered cube, "
+                               "10,368,000 tets, 9-comp strain IP field, IP order 2",
+                   "form": args.form, "tets": T, "vertices": V, "edges": E, "patches": patches},
+        "phases_ms": {"fit": phases[0], "expand": phases[1], "expanded_fit": phases[2],
+                      "element": phases[3], "factor_average": phases[4]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "recovery (fit + expand + expanded fit), %d algorithmic bytes"
+                               % ab["recover"]},
+    }
+    if args.form != "fp64" and not args.no_compare:
+        # accuracy against the FP64 form (bit-identical to the FP64 oracle)
+        s_fast = spr.getSPRSizeField(eps, 0.05, form=args.form)
+        a_fast = s_fast.data[:, 0].clone()
+        m.destroyField(s_fast)
+        s64 = spr.getSPRSizeField(eps, 0.05, form="fp64")
+        fp64_ms = float(m.phase_times().sum())
+        a64 = s64.data[:, 0].clone()
+        m.destroyField(s64)
+        rel = (a_fast / a64 - 1).abs()
+        r_fast = spr.recoverField(eps, form=args.form)
+        rf = r_fast.data.clone()
+        m.destroyField(r_fast)
+        r64 = spr.recoverField(eps, form="fp64")
+        scale = r64.data.abs().amax(0)
+        keep = scale > 0
+        rr = ((rf - r64.data).abs()[:, keep] / scale[keep])
+        m.destroyField(r64)
+        out["accuracy_vs_fp64"] = {
+            "size_max_rel": float(rel.max()), "size_rms_rel": float(rel.pow(2).mean().sqrt()),
+            "recovered_max_rel": float(rr.max()), "recovered_rms_rel": float(rr.pow(2).mean().sqrt()),
+            "fp64_form_gpu_ms": fp64_ms,
+            "speedup_vs_fp64_form": fp64_ms / float(phases.sum())}
+        log("accuracy vs fp64: %s" % out["accuracy_vs_fp64"])
+    if not args.no_cpu:
+        log("cpu baseline (budget %.0f s)" % args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(host, ip, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -19,6 +19,7 @@ SPRG_EINTERNAL = 6
 
 SPRG_FP64 = 0
 SPRG_FP32R = 1
+SPRG_FP64F = 2
 
 _c_int = ctypes.c_int
 _c_i64 = ctypes.c_int64
@@ -42,6 +43,7 @@ SIGNATURES = {
     "sprg_mesh_destroy": (_c_int, [_vp]),
     "sprg_mesh_counts": (_c_int, [_vp, _vp]),
     "sprg_mesh_adjacency": (_c_int, [_vp, _c_int, _pp, _pp]),
+    "sprg_mesh_phase_times": (_c_int, [_vp, _vp, _c_int]),
     "sprg_grad_ip": (_c_int, [_vp, _c_int, _c_int, _vp, _c_int, _vp]),
     "sprg_patch_sets": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp]),
     "sprg_recover": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int]),

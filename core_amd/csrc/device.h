@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
+
 #include "sprgpu_internal.h"
 
 namespace sprg {
@@ -43,14 +45,18 @@ enum {
   ST_PATCH_FAIL = 1,     // "all hope is lost"
   ST_TOO_LARGE = 2,      // a patch exceeded the supported row count
   ST_XSET_OVERFLOW = 4,  // expanded-set storage too small
-  ST_INTERNAL = 8
+  ST_INTERNAL = 8,
+  ST_ILLCOND = 16        // a patch too ill-conditioned for the FP64 fallback
 };
 
 struct Lists {
   int32_t* items[NBUCKET];   // regular: patch ids; expanded: item ids
   int32_t* count;            // [NBUCKET] regular, [NBUCKET] expanded, [1] expand-queue
   int32_t* xitems[NBUCKET];
-  int32_t* xq;               // patches queued for expansion
+  int32_t* xq;               // patches queued for the exact rank test /
+                             // expansion; bit 31: test the initial set first
+  int32_t* fbq;              // fast forms: FP64 fallback queue; bit 31: an
+                             // expanded item rather than a patch id
   // expanded sets
   int32_t* x_patch;
   int64_t* x_off;
@@ -64,6 +70,7 @@ struct Lists {
 
 struct FitArgs {
   MeshView m;
+  int fast;        // fast forms: one regular list, one expanded list
   int q;           // points per element
   int nc;          // components
   int64_t npatch;  // nv (+ ne)
@@ -79,6 +86,7 @@ int launch_scan_i32(int32_t* a, int64_t n, int32_t* work, hipStream_t s);
 int launch_classify(const FitArgs& a, int nterms, hipStream_t s);
 int launch_exact_fit(const FitArgs& a, int nterms, bool expanded, hipStream_t s);
 int launch_expand(const FitArgs& a, int nterms, hipStream_t s);
+int launch_fast_fit(const FitArgs& a, int nterms, int form, int src, hipStream_t s);
 int launch_grad(const MeshView& m, int field_order, int nc_in,
                 const double* nodal, int ip_order, double* ip, hipStream_t s);
 
@@ -96,7 +104,9 @@ struct EstArgs {
   double* size;        // [nv]
   int nblocks;
 };
-int launch_estimate(EstArgs& a, hipStream_t s);
+// mark(i) records phase event i on the stream (3: before the element pass,
+// 4: after it, 5: after the vertex average).
+int launch_estimate(EstArgs& a, hipStream_t s, const std::function<void(int)>& mark);
 
 }  // namespace sprg
 

@@ -183,10 +183,11 @@ __global__ void k_average(EstArgs a)
 
 }  // namespace
 
-int launch_estimate(EstArgs& a, hipStream_t s)
+int launch_estimate(EstArgs& a, hipStream_t s, const std::function<void(int)>& mark)
 {
   if (a.m.nt == 0) return sprg_fail(SPRG_EINVAL, "estimate: empty mesh");
   a.nblocks = (int)((a.m.nt + EST_T - 1) / EST_T);
+  mark(3);
   switch (a.nc) {
     case 1: k_element<1><<<a.nblocks, EST_T, 0, s>>>(a); break;
     case 3: k_element<3><<<a.nblocks, EST_T, 0, s>>>(a); break;
@@ -194,8 +195,10 @@ int launch_estimate(EstArgs& a, hipStream_t s)
     case 27: k_element<27><<<a.nblocks, EST_T, 0, s>>>(a); break;
     default: return sprg_fail(SPRG_EUNSUPPORTED, "estimate: %d components", a.nc);
   }
+  mark(4);
   k_factor<<<1, EST_T, 0, s>>>(a);
   k_average<<<(unsigned)((a.m.nv + 255) / 256), 256, 0, s>>>(a);
+  mark(5);
   return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "estimate launch failed");
 }
 

@@ -130,28 +130,50 @@ __device__ __forceinline__ void patch_set(const FitArgs& a, int64_t p,
   }
 }
 
+// Appends v to list[*counter] with one atomic per wave and target list.
+__device__ __forceinline__ void wave_append(int target, int my_target, int32_t* count,
+                                            int32_t* list, int32_t v)
+{
+  const uint64_t mask = __ballot(my_target == target);
+  if (!mask) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(mask));
+  base = __shfl(base, leader);
+  if (my_target == target) {
+    const uint64_t below = mask & ((1ull << lane) - 1);
+    list[base + __popcll(below)] = v;
+  }
+}
+
 // Sorts patches into row-count buckets; patches with fewer rows than terms
-// go straight to the expansion queue (hasEnoughPoints, :355-363).
-__global__ void k_classify(FitArgs a, int nterms)
+// go straight to the expansion queue (hasEnoughPoints, :355-363).  With
+// a.fast set, every regular patch goes to list 0.
+__global__ __launch_bounds__(256) void k_classify(FitArgs a, int nterms)
 {
   int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= a.npatch) return;
-  const int32_t* set;
-  int ns;
-  patch_set(a, p, &set, &ns);
-  int m = ns * a.q;
-  if (m < nterms) {
-    int k = atomicAdd(a.L.count + 2 * NBUCKET, 1);
-    a.L.xq[k] = (int32_t)p;
-    return;
+  int target = -1;  // -1 none, 0..NBUCKET-1 bucket, NBUCKET expansion queue
+  if (p < a.npatch) {
+    const int32_t* set;
+    int ns;
+    patch_set(a, p, &set, &ns);
+    int m = ns * a.q;
+    if (m < nterms) {
+      target = NBUCKET;
+    } else if (a.fast) {
+      target = 0;
+    } else {
+      target = bucket_of(m);
+      if (target >= NBUCKET) {
+        atomicOr(a.L.status, ST_TOO_LARGE);
+        target = -1;
+      }
+    }
   }
-  int b = bucket_of(m);
-  if (b >= NBUCKET) {
-    atomicOr(a.L.status, ST_TOO_LARGE);
-    return;
-  }
-  int k = atomicAdd(a.L.count + b, 1);
-  a.L.items[b][k] = (int32_t)p;
+  for (int b = 0; b < NBUCKET; ++b)
+    wave_append(b, target, a.L.count + b, a.L.items[b], (int32_t)p);
+  wave_append(NBUCKET, target, a.L.count + 2 * NBUCKET, a.L.xq, (int32_t)p);
 }
 
 template <int NT, int MMAX, int LANES, bool EXP>
@@ -311,7 +333,9 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
   const int q = a.q;
   const int nq = a.L.count[2 * NBUCKET];
   for (int it = blockIdx.x; it < nq; it += gridDim.x) {
-    const int64_t p = a.L.xq[it];
+    const int32_t entry = a.L.xq[it];
+    const int64_t p = entry & 0x7fffffff;
+    const bool test_first = entry < 0;
     const int32_t* set0;
     int ns0;
     patch_set(a, p, &set0, &ns0);
@@ -325,6 +349,18 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
     for (int i = lane; i < ns0; i += 64) sCur[i] = set0[i];
     __syncthreads();
     bool done = false, failed = false;
+    if (test_first && ns0 * q >= NT) {
+      // a fast form could not certify the rank: run the reference's test
+      const int mrows = ns0 * q;
+      if (mrows > XMAXR) {
+        failed = true;
+        if (lane == 0) sOverflow = 1;
+      } else {
+        build_rows<NT, 64>(m, q, sCur, mrows, sR, lane);
+        __syncthreads();
+        done = r_recurrence<NT, 64, XMAXR>(sR, sV, sDot, sNorm, mrows, lane, true);
+      }
+    }
     for (int round = 0; round < 64 && !done && !failed; ++round) {
       const int nold = sN;
       for (int i = lane; i < nold; i += 64) sOld[i] = sCur[i];
@@ -414,7 +450,7 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
           a.L.x_patch[sItem] = (int32_t)p;
           a.L.x_off[sItem] = sOff;
           a.L.x_len[sItem] = ns;
-          int b = bucket_of(ns * q);
+          int b = a.fast ? 0 : bucket_of(ns * q);
           int k = atomicAdd(a.L.count + NBUCKET + b, 1);
           a.L.xitems[b][k] = sItem;
         }

@@ -1,0 +1,313 @@
+// k_fast.hip -- the fast forms of the patch recovery: one thread per patch,
+// normal equations in a node-centred monomial basis, and the recovered value
+// as a weighted sum of the patch's samples.
+//
+// Recovered value at the patch node x_n (spr/sprRecoverField.cc:304-353):
+//   v_c = phi(x_n)^T (A^T A)^-1 A^T b_c.
+// With monomials of d = x - x_n, phi(x_n) = e_0, so with z = G^-1 e_0
+// (G = A^T A) every component is v_c = sum_i w_i b_ic, w_i = phi_i . z,
+// and since sum_i w_i = 1 it is evaluated as b_0c + sum_i w_i (b_ic - b_0c),
+// whose rounding scales with the variation of b over the patch rather
+// than with |b|.  Centring is a triangular change of the graded monomial
+// basis, so the fit (and the diagonal of its Cholesky factor, which equals
+// the reference's Householder R_kk on raw coordinates in exact arithmetic)
+// is unchanged.
+//
+//   T = float : G and its Cholesky factor in FP32; NREF (3) residual
+//               refinement steps r = e_0 - A^T (A z) in FP64;
+//   T = double: G and the factor in FP64 (NREF = 0), or the FP64 fallback
+//               for ill-conditioned patches (NREF = 2).
+//
+// Rank: the reference rejects a patch when a Householder column norm falls
+// below 1e-10 (mth/mthQR.cc:31).  The Cholesky pivot L_kk is that norm; a
+// patch is certified full rank here only if every pivot clears 1e-10 with a
+// margin and was computed without heavy cancellation (pivot / diagonal >=
+// tau_rank).  Anything else goes to the exact FP64 path (k_expand tests the
+// initial set with the reference's arithmetic before growing it).  Certified
+// patches whose smallest pivot ratio is below tau_acc go to the FP64
+// fallback.  Three passes over the rows: Gram, refinement, weighted sum;
+// sample positions are recomputed in each (affine in the four vertices for a
+// straight-sided mesh, the Lagrange-2 map otherwise).
+#include "device.h"
+#include "spr_math.h"
+
+namespace sprg {
+namespace {
+
+constexpr int NQ_MAX = 4;  // points per element the fast kernels handle
+
+struct Geo {
+  double x[10][3];
+};
+
+__device__ __forceinline__ void load_geo(const MeshView& m, int32_t t, bool p2map, Geo& g)
+{
+  const int64_t tt = t;
+  for (int a = 0; a < 4; ++a) {
+    const double* p = m.xyz + 3 * (int64_t)m.tv[4 * tt + a];
+    g.x[a][0] = p[0]; g.x[a][1] = p[1]; g.x[a][2] = p[2];
+  }
+  if (p2map)
+    for (int e = 0; e < 6; ++e) {
+      const double* p = m.xyz + 3 * (m.nv + (int64_t)m.te[6 * tt + e]);
+      g.x[4 + e][0] = p[0]; g.x[4 + e][1] = p[1]; g.x[4 + e][2] = p[2];
+    }
+}
+
+// Sample point l of the mesh-order rule, minus the node position.
+__device__ __forceinline__ void sample_d(const MeshView& m, const Geo& g, bool p2map,
+                                         int l, const double* xn, double* d)
+{
+  double xi[3];
+  rule_point(m.order, l, xi);
+  double x[3];
+  if (p2map) {
+    double N[10];
+    tet_values(2, xi, N);
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int k = 0; k < 10; ++k) s = fma(N[k], g.x[k][j], s);
+      x[j] = s;
+    }
+  } else {
+    const double l0 = 1 - xi[0] - xi[1] - xi[2];
+    for (int j = 0; j < 3; ++j)
+      x[j] = fma(xi[2], g.x[3][j], fma(xi[1], g.x[2][j], fma(xi[0], g.x[1][j], l0 * g.x[0][j])));
+  }
+  d[0] = x[0] - xn[0];
+  d[1] = x[1] - xn[1];
+  d[2] = x[2] - xn[2];
+}
+
+template <class S, int NT>
+__device__ __forceinline__ void basis(const double* d, S* f)
+{
+  const S a = (S)d[0], b = (S)d[1], c = (S)d[2];
+  f[0] = 1;
+  f[1] = a;
+  f[2] = b;
+  f[3] = c;
+  if (NT == 10) {
+    f[4] = a * b;
+    f[5] = b * c;
+    f[6] = c * a;
+    f[7] = a * a;
+    f[8] = b * b;
+    f[9] = c * c;
+  }
+}
+
+// packed lower triangle index
+__host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+
+template <class T, int NT>
+__device__ __forceinline__ void chol_solve(const T* L, const T* rhs, T* z)
+{
+  T y[NT];
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    T s = rhs[k];
+#pragma unroll
+    for (int j = 0; j < k; ++j) s = fma(-L[tri(k, j)], y[j], s);
+    y[k] = s / L[tri(k, k)];
+  }
+#pragma unroll
+  for (int kk = 0; kk < NT; ++kk) {
+    const int k = NT - 1 - kk;
+    T s = y[k];
+#pragma unroll
+    for (int j = k + 1; j < NT; ++j) s = fma(-L[tri(j, k)], z[j], s);
+    z[k] = s / L[tri(k, k)];
+  }
+}
+
+enum { SRC_REGULAR = 0, SRC_EXPANDED = 1, SRC_FALLBACK = 2 };
+
+template <class T, int NT, int NC, int NREF>
+__global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank, float tau_acc)
+{
+  const MeshView& m = a.m;
+  const int q = a.q;
+  const bool p2map = m.order == 2 && !m.straight;
+  const int32_t* list = src == SRC_REGULAR ? a.L.items[0]
+                        : src == SRC_EXPANDED ? a.L.xitems[0] : a.L.fbq;
+  const int count = src == SRC_REGULAR ? a.L.count[0]
+                    : src == SRC_EXPANDED ? a.L.count[NBUCKET] : a.L.count[2 * NBUCKET + 1];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < count; it += stride) {
+    const int32_t entry = list[it];
+    bool expanded = src == SRC_EXPANDED || (src == SRC_FALLBACK && entry < 0);
+    const int32_t id = entry & 0x7fffffff;
+    int64_t p;
+    const int32_t* set;
+    int ns;
+    if (expanded) {
+      p = a.L.x_patch[id];
+      set = a.L.x_set + a.L.x_off[id];
+      ns = a.L.x_len[id];
+    } else {
+      p = id;
+      if (p < m.nv) {
+        set = m.v2t + m.v2t_off[p];
+        ns = m.v2t_off[p + 1] - m.v2t_off[p];
+      } else {
+        set = m.e2t + m.e2t_off[p - m.nv];
+        ns = m.e2t_off[p - m.nv + 1] - m.e2t_off[p - m.nv];
+      }
+    }
+    const double xn[3] = {m.xyz[3 * p], m.xyz[3 * p + 1], m.xyz[3 * p + 2]};
+    // pass 1: Gram matrix, packed lower triangle
+    constexpr int NG = NT * (NT + 1) / 2;
+    T G[NG];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) G[k] = 0;
+    for (int s = 0; s < ns; ++s) {
+      Geo g;
+      load_geo(m, set[s], p2map, g);
+      for (int l = 0; l < q; ++l) {
+        double d[3];
+        sample_d(m, g, p2map, l, xn, d);
+        T f[NT];
+        basis<T, NT>(d, f);
+#pragma unroll
+        for (int i = 0; i < NT; ++i)
+#pragma unroll
+          for (int j = 0; j <= i; ++j) G[tri(i, j)] = fma(f[i], f[j], G[tri(i, j)]);
+      }
+    }
+    // Cholesky in place, with the rank certificate and the conditioning
+    bool rank_ok = true;
+    float min_ratio = 1.0f;
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const T diag = G[tri(k, k)];
+      T dk = diag;
+#pragma unroll
+      for (int j = 0; j < k; ++j) dk = fma(-G[tri(k, j)], G[tri(k, j)], dk);
+      const float ratio = (diag > 0 && dk > 0) ? (float)(dk / diag) : 0.0f;
+      min_ratio = fminf(min_ratio, ratio);
+      const T lkk = dk > 0 ? sqrt(dk) : (T)0;
+      if (!(ratio >= tau_rank) || !((double)lkk >= 1.05e-10)) rank_ok = false;
+      G[tri(k, k)] = lkk > 0 ? lkk : (T)1;
+#pragma unroll
+      for (int i = k + 1; i < NT; ++i) {
+        T s = G[tri(i, k)];
+#pragma unroll
+        for (int j = 0; j < k; ++j) s = fma(-G[tri(i, j)], G[tri(k, j)], s);
+        G[tri(i, k)] = s / G[tri(k, k)];
+      }
+    }
+    if (src == SRC_REGULAR && !rank_ok) {
+      // not certified: the exact path decides (test the initial set first)
+      int k = atomicAdd(a.L.count + 2 * NBUCKET, 1);
+      a.L.xq[k] = (int32_t)(p | 0x80000000u);
+      continue;
+    }
+    if (!(min_ratio >= tau_acc)) {
+      if (src == SRC_FALLBACK) {
+        atomicOr(a.L.status, ST_ILLCOND);
+      } else {
+        int k = atomicAdd(a.L.count + 2 * NBUCKET + 1, 1);
+        a.L.fbq[k] = expanded ? (int32_t)(id | 0x80000000u) : (int32_t)p;
+      }
+      continue;
+    }
+    // z = G^-1 e_0, refined against the FP64 residual
+    T e0[NT], zt[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) e0[k] = k == 0 ? 1 : 0;
+    chol_solve<T, NT>(G, e0, zt);
+    double z[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) z[k] = (double)zt[k];
+    for (int r = 0; r < NREF; ++r) {
+      double u[NT];
+#pragma unroll
+      for (int k = 0; k < NT; ++k) u[k] = 0;
+      for (int s = 0; s < ns; ++s) {
+        Geo g;
+        load_geo(m, set[s], p2map, g);
+        for (int l = 0; l < q; ++l) {
+          double d[3], f[NT];
+          sample_d(m, g, p2map, l, xn, d);
+          basis<double, NT>(d, f);
+          double t = 0;
+#pragma unroll
+          for (int k = 0; k < NT; ++k) t = fma(f[k], z[k], t);
+#pragma unroll
+          for (int k = 0; k < NT; ++k) u[k] = fma(f[k], t, u[k]);
+        }
+      }
+      T res[NT], dz[NT];
+#pragma unroll
+      for (int k = 0; k < NT; ++k) res[k] = (T)((k == 0 ? 1.0 : 0.0) - u[k]);
+      chol_solve<T, NT>(G, res, dz);
+#pragma unroll
+      for (int k = 0; k < NT; ++k) z[k] += (double)dz[k];
+    }
+    // weighted sum, centred on the first sample
+    double v[NC], b0[NC];
+    {
+      const double* bp = a.ip + (int64_t)set[0] * q * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) { b0[c] = bp[c]; v[c] = 0; }
+    }
+    for (int s = 0; s < ns; ++s) {
+      Geo g;
+      const int32_t t = set[s];
+      load_geo(m, t, p2map, g);
+      for (int l = 0; l < q; ++l) {
+        double d[3], f[NT];
+        sample_d(m, g, p2map, l, xn, d);
+        basis<double, NT>(d, f);
+        double w = 0;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) w = fma(f[k], z[k], w);
+        const double* bp = a.ip + ((int64_t)t * q + l) * NC;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c] = fma(w, bp[c] - b0[c], v[c]);
+      }
+    }
+    double* out = a.out + p * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = b0[c] + v[c];
+  }
+}
+
+template <class T, int NT, int NREF>
+int launch_nc(const FitArgs& a, int src, float tau_rank, float tau_acc, hipStream_t s)
+{
+  const unsigned blocks = 4096, threads = 256;
+  switch (a.nc) {
+    case 1: k_fast<T, NT, 1, NREF><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
+    case 3: k_fast<T, NT, 3, NREF><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
+    case 9: k_fast<T, NT, 9, NREF><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
+    case 27: k_fast<T, NT, 27, NREF><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
+    default: return sprg_fail(SPRG_EUNSUPPORTED, "fast recovery: %d components", a.nc);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "fast fit launch failed");
+}
+
+template <class T, int NREF>
+int launch_t(const FitArgs& a, int nterms, int src, float tau_rank, float tau_acc, hipStream_t s)
+{
+  return nterms == 4 ? launch_nc<T, 4, NREF>(a, src, tau_rank, tau_acc, s)
+                     : launch_nc<T, 10, NREF>(a, src, tau_rank, tau_acc, s);
+}
+
+}  // namespace
+
+// form SPRG_FP32R: FP32 Gram + 3 FP64 refinement steps; SPRG_FP64F: FP64
+// Gram, no refinement.  src: 0 regular, 1 expanded,
+// 2 the FP64 fallback queue (FP64 Gram + 2 refinement steps).
+int launch_fast_fit(const FitArgs& a, int nterms, int form, int src, hipStream_t s)
+{
+  if (a.q > NQ_MAX) return sprg_fail(SPRG_EUNSUPPORTED, "fast recovery: %d points per element", a.q);
+  if (src == 2) return launch_t<double, 2>(a, nterms, src, 0.0f, 1e-13f, s);
+  // FP32: three FP64 refinement steps; each contracts the error by about
+  // kappa(G) * 2^-24, and the size field needs the recovered field to ~1e-12
+  if (form == SPRG_FP32R) return launch_t<float, 3>(a, nterms, src, 1e-4f, 1e-3f, s);
+  return launch_t<double, 0>(a, nterms, src, 1e-11f, 1e-6f, s);
+}
+
+}  // namespace sprg

@@ -11,6 +11,11 @@
 
 using namespace sprg;
 
+// Phases timed with events: 0 classify + fit of the initial patch sets,
+// 1 expansion, 2 fit of the expanded sets, 3 element pass, 4 size factor and
+// vertex average.
+constexpr int NPHASE = 5;
+
 struct sprg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -56,6 +61,16 @@ struct sprg_mesh {
   // per-call workspace
   DBuf lists, xset, rec, elem, partial, scalars, status, setlen;
   int64_t list_cap = 0, xset_cap = 0;
+  // phase timing of the last call: events on the context stream
+  hipEvent_t ev[NPHASE + 1] = {};
+  bool ev_ok = false;
+  double phase_ms[NPHASE] = {};
+  int phase_last = 0;  // highest phase recorded by the last call
+  void mark(int i)
+  {
+    if (ev_ok) hipEventRecord(ev[i], ctx->stream);
+    if (i > phase_last) phase_last = i;
+  }
 
   MeshView view() const
   {
@@ -190,6 +205,9 @@ extern "C" int sprg_mesh_create(sprg_ctx* ctx, int64_t nv, int64_t nt, int64_t n
   sprg_mesh* m = new (std::nothrow) sprg_mesh;
   if (!m) return sprg_fail(SPRG_ENOMEM, "mesh_create: out of host memory");
   m->ctx = ctx;
+  m->ev_ok = true;
+  for (int i = 0; i <= NPHASE; ++i)
+    if (hipEventCreate(&m->ev[i]) != hipSuccess) m->ev_ok = false;
   m->nv = nv;
   m->nt = nt;
   m->ne = tet_edges ? ne : 0;
@@ -251,6 +269,8 @@ extern "C" int sprg_mesh_destroy(sprg_mesh* m)
                  &m->lists, &m->xset, &m->rec, &m->elem, &m->partial, &m->scalars,
                  &m->status, &m->setlen};
   for (DBuf* b : all) b->release();
+  for (int i = 0; i <= NPHASE; ++i)
+    if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
   delete m;
   return 0;
 }
@@ -263,6 +283,13 @@ extern "C" int sprg_mesh_counts(sprg_mesh* m, int64_t* counts)
   counts[2] = m->ne;
   counts[3] = m->nnodes;
   return 0;
+}
+
+extern "C" int sprg_mesh_phase_times(sprg_mesh* m, double* ms, int n)
+{
+  if (!m || !ms) return sprg_fail(SPRG_EINVAL, "phase_times: null argument");
+  for (int i = 0; i < n; ++i) ms[i] = i < NPHASE ? m->phase_ms[i] : 0;
+  return NPHASE;
 }
 
 extern "C" int sprg_mesh_adjacency(sprg_mesh* m, int dim, const int32_t** off, const int32_t** idx)
@@ -298,6 +325,17 @@ int check_ip(sprg_mesh* m, int ip_order, int nc, int* q)
   return 0;
 }
 
+void collect_phases(sprg_mesh* m)
+{
+  for (int i = 0; i < NPHASE; ++i) {
+    float ms = 0;
+    m->phase_ms[i] = 0;
+    if (m->ev_ok && i < m->phase_last &&
+        hipEventElapsedTime(&ms, m->ev[i], m->ev[i + 1]) == hipSuccess)
+      m->phase_ms[i] = ms;
+  }
+}
+
 // Lays out the work lists in one device allocation.
 int setup_lists(sprg_mesh* m, FitArgs& a)
 {
@@ -305,7 +343,7 @@ int setup_lists(sprg_mesh* m, FitArgs& a)
   if (m->list_cap < np) {
     // items and xitems per bucket, xq, x_patch, x_len: int32; x_off: int64;
     // counts, cursors, status
-    size_t n32 = (size_t)np * (2 * NBUCKET + 3) + 2 * NBUCKET + 8;
+    size_t n32 = (size_t)np * (2 * NBUCKET + 4) + 2 * NBUCKET + 8;
     size_t bytes = n32 * 4 + (size_t)np * 8 + 64;
     int rc = m->lists.ensure(bytes);
     if (rc) return rc;
@@ -331,6 +369,7 @@ int setup_lists(sprg_mesh* m, FitArgs& a)
   for (int b = 0; b < NBUCKET; ++b) { L.items[b] = p; p += m->list_cap; }
   for (int b = 0; b < NBUCKET; ++b) { L.xitems[b] = p; p += m->list_cap; }
   L.xq = p; p += m->list_cap;
+  L.fbq = p; p += m->list_cap;
   L.x_patch = p; p += m->list_cap;
   L.x_len = p; p += m->list_cap;
   L.x_set = m->xset.as<int32_t>();
@@ -351,13 +390,17 @@ int read_status(sprg_mesh* m, const FitArgs& a, int32_t* st)
 // expanded sets.  Retries once with larger expanded-set storage.
 int run_recover(sprg_mesh* m, int q, int nc, const double* ip, double* out, int form)
 {
-  if (form != SPRG_FP64)
-    return sprg_fail(SPRG_EUNSUPPORTED, "form %d is not available in this build", form);
+  if (form != SPRG_FP64 && form != SPRG_FP32R && form != SPRG_FP64F)
+    return sprg_fail(SPRG_EINVAL, "unknown numeric form %d", form);
+  const bool fast = form != SPRG_FP64;
+  if (fast && q > 4)
+    return sprg_fail(SPRG_EUNSUPPORTED, "the fast forms take at most 4 points per element");
   const int nterms = m->order == 1 ? 4 : 10;
   hipStream_t s = m->ctx->stream;
   for (int attempt = 0; attempt < 4; ++attempt) {
     FitArgs a;
     a.m = m->view();
+    a.fast = fast ? 1 : 0;
     a.q = q;
     a.nc = nc;
     a.npatch = m->npatch();
@@ -367,12 +410,27 @@ int run_recover(sprg_mesh* m, int q, int nc, const double* ip, double* out, int 
     if (rc) return rc;
     HIP_OK(hipMemsetAsync(a.L.x_cursor, 0, sizeof(int64_t) * 2, s), "memset");
     HIP_OK(hipMemsetAsync(a.L.count, 0, sizeof(int32_t) * (2 * NBUCKET + 4), s), "memset");
+    m->phase_last = 0;
+    m->mark(0);
     if ((rc = launch_classify(a, nterms, s))) return rc;
-    if ((rc = launch_exact_fit(a, nterms, false, s))) return rc;
+    if (fast) {
+      if ((rc = launch_fast_fit(a, nterms, form, 0, s))) return rc;
+    } else {
+      if ((rc = launch_exact_fit(a, nterms, false, s))) return rc;
+    }
+    m->mark(1);
     if ((rc = launch_expand(a, nterms, s))) return rc;
-    if ((rc = launch_exact_fit(a, nterms, true, s))) return rc;
+    m->mark(2);
+    if (fast) {
+      if ((rc = launch_fast_fit(a, nterms, form, 1, s))) return rc;
+      if ((rc = launch_fast_fit(a, nterms, form, 2, s))) return rc;
+    } else {
+      if ((rc = launch_exact_fit(a, nterms, true, s))) return rc;
+    }
+    m->mark(3);
     int32_t st = 0;
     if ((rc = read_status(m, a, &st))) return rc;
+    collect_phases(m);
     if (st & ST_XSET_OVERFLOW) {
       int64_t cap = 2 * m->xset_cap;
       if ((rc = m->xset.ensure(sizeof(int32_t) * cap))) return rc;
@@ -383,6 +441,9 @@ int run_recover(sprg_mesh* m, int q, int nc, const double* ip, double* out, int 
     if (st & ST_TOO_LARGE)
       return sprg_fail(SPRG_EUNSUPPORTED, "SPR: a patch needs more than %d sample rows", bucket_rows(NBUCKET - 1));
     if (st & ST_INTERNAL) return sprg_fail(SPRG_EINTERNAL, "SPR: inconsistent rank decision");
+    if (st & ST_ILLCOND)
+      return sprg_fail(SPRG_EUNSUPPORTED,
+                       "SPR: a patch is too ill-conditioned for the fast forms; use SPRG_FP64");
     return 0;
   }
   return sprg_fail(SPRG_ENOMEM, "SPR: expanded patch storage kept overflowing");
@@ -453,11 +514,12 @@ int run_estimate(sprg_mesh* m, int ip_order, int nc, const double* ip, int targe
   e.scalars = m->scalars.as<double>();
   e.size = size;
   e.nblocks = (int)nblocks;
-  if ((rc = launch_estimate(e, m->ctx->stream))) return rc;
+  if ((rc = launch_estimate(e, m->ctx->stream, [m](int i) { m->mark(i); }))) return rc;
   double sc[4] = {0, 0, 0, 0};
   HIP_OK(hipMemcpyAsync(sc, e.scalars, sizeof(double) * 3, hipMemcpyDeviceToHost, m->ctx->stream),
          "scalars");
   HIP_OK(hipStreamSynchronize(m->ctx->stream), "estimate sync");
+  collect_phases(m);
   auto t1 = std::chrono::steady_clock::now();
   if (diag) {
     if (!target) {
@@ -557,6 +619,7 @@ extern "C" int sprg_patch_sets(sprg_mesh* m, int ip_order, int64_t* set_off, int
   if ((rc = run_recover(m, q, 0, m->rec.as<double>(), m->rec.as<double>(), SPRG_FP64))) return rc;
   FitArgs a;
   a.m = m->view();
+  a.fast = 0;
   a.q = q;
   a.nc = 0;
   a.npatch = m->npatch();

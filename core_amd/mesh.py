@@ -53,8 +53,10 @@ class HostMesh:
         straight-sided mesh up to rounding)."""
         xi, _ = tet_rule(ip_order)
         lam = np.stack([1 - xi.sum(1), xi[:, 0], xi[:, 1], xi[:, 2]], 1)
-        X = self.xyz[self.tv]
-        return np.einsum("qa,tak->tqk", lam, X)
+        P = np.zeros((self.nt, lam.shape[0], 3))
+        for a in range(4):
+            P += lam[None, :, a, None] * self.xyz[self.tv[:, a]][:, None, :]
+        return P
 
 
 def tet_rule(order):

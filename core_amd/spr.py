@@ -21,11 +21,11 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import SPRError, SPRG_FP32R, SPRG_FP64
+from ._lib import SPRError, SPRG_FP32R, SPRG_FP64, SPRG_FP64F
 from .mesh import HostMesh
 
 SCALAR, VECTOR, MATRIX = 1, 3, 9
-FORMS = {"fp64": SPRG_FP64, "fp32r": SPRG_FP32R}
+FORMS = {"fp64": SPRG_FP64, "fp32r": SPRG_FP32R, "fp64f": SPRG_FP64F}
 _CTX = {}
 
 
@@ -145,6 +145,13 @@ class Mesh:
         if i.size:
             _lib.check(L.sprg_download(self.ctx, i.ctypes.data, idx, i.nbytes))
         return o, i
+
+    def phase_times(self):
+        """GPU ms of the last call's phases: fit, expand, expanded fit,
+        element pass, factor + average (HIP events on the library stream)."""
+        ms = np.zeros(5)
+        _lib.lib().sprg_mesh_phase_times(self.handle, ms.ctypes.data, 5)
+        return ms
 
     def patch_sets(self, ip_order):
         """(off, idx) of the tet set every recovered entity is fitted on."""

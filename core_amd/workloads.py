@@ -111,5 +111,17 @@ def strain_mesh(n=120, jitter=0.25, seed=C5):
     return _mesh.box(n, order=2, jitter=jitter, seed=seed)
 
 
-def strain_ip(m, ip_order=2):
-    return strain_field(m.ip_points(ip_order))
+def strain_ip(m, ip_order=2, chunk=1 << 20, progress=None):
+    """The strain field at the IP positions, computed in chunks of tets."""
+    xi, _ = _mesh.tet_rule(ip_order)
+    lam = np.stack([1 - xi.sum(1), xi[:, 0], xi[:, 1], xi[:, 2]], 1)
+    out = np.empty((m.nt, lam.shape[0], 9))
+    for t0 in range(0, m.nt, chunk):
+        tv = m.tv[t0:t0 + chunk]
+        P = np.zeros((tv.shape[0], lam.shape[0], 3))
+        for a in range(4):
+            P += lam[None, :, a, None] * m.xyz[tv[:, a]][:, None, :]
+        out[t0:t0 + chunk] = strain_field(P)
+        if progress:
+            progress(min(t0 + chunk, m.nt), m.nt)
+    return out

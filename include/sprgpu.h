@@ -48,8 +48,13 @@ enum {
 
 /* Numeric form of the recovery and estimation kernels. */
 enum {
-  SPRG_FP64 = 0,  /* plain FP64 form */
-  SPRG_FP32R = 1  /* FP32 patch solves + one FP64 residual refinement step */
+  SPRG_FP64 = 0,   /* plain FP64 form: the reference's Householder QR with
+                      explicit Q, operation for operation (bit-identical
+                      recovered field) */
+  SPRG_FP32R = 1,  /* FP32 normal equations in a node-centred basis + FP64
+                      residual refinement (3 steps); FP64 fallback for
+                      ill-conditioned patches */
+  SPRG_FP64F = 2   /* the same fast algorithm with FP64 normal equations */
 };
 
 typedef struct sprg_ctx sprg_ctx;
@@ -85,6 +90,13 @@ int sprg_mesh_create(sprg_ctx* ctx, int64_t nv, int64_t nt, int64_t ne,
 int sprg_mesh_destroy(sprg_mesh* mesh);
 /* counts[0..3] = nv, nt, ne, nodes */
 int sprg_mesh_counts(sprg_mesh* mesh, int64_t* counts);
+/* GPU time (ms, HIP events on the context stream) of the phases of the last
+ * recover / size-field call on this mesh: [0] fit of the initial patch sets,
+ * [1] patch expansion, [2] fit of the expanded sets, [3] element pass,
+ * [4] size factor + vertex average (0 for phases the call did not run).
+ * Writes min(n, 5) values and returns 5.  No reference counterpart (the
+ * reference times the whole call, spr/sprEstimateError.cc:380-386). */
+int sprg_mesh_phase_times(sprg_mesh* mesh, double* ms, int n);
 /* Device CSR of getAdjacent(entity, 3): dim 0 (vertices) or 1 (edges).
  * off has n+1 entries, idx has off[n]; both stay owned by the mesh. */
 int sprg_mesh_adjacency(sprg_mesh* mesh, int dim, const int32_t** off,

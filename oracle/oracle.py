@@ -192,3 +192,13 @@ def current_size(mesh):
     h = np.zeros(mesh.nt)
     _check(lib().om_current_size(mesh.c(), _ptr(h)))
     return h
+
+
+def element_error_range(mesh, ip_order, ip, rec, t0, t1):
+    ip = np.ascontiguousarray(ip, dtype=np.float64)
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    s = np.zeros(mesh.nt)
+    f = lib().om_element_error_range
+    f.argtypes = [_p, ctypes.c_int, ctypes.c_int, _p, _p, _p, _i64, _i64]
+    _check(f(mesh.c(), ip_order, ip.shape[2], _ptr(ip), _ptr(rec), _ptr(s), t0, t1))
+    return s

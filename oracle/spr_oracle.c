@@ -758,7 +758,15 @@ static double element_error(const om_mesh* m, int ip_order, int nc,
 int om_element_error(const om_mesh* m, int ip_order, int nc, const double* ip,
                      const double* rec, double* s)
 {
-  for (int64_t t = 0; t < m->nt; ++t)
+  return om_element_error_range(m, ip_order, nc, ip, rec, s, 0, m->nt);
+}
+
+int om_element_error_range(const om_mesh* m, int ip_order, int nc,
+                           const double* ip, const double* rec, double* s,
+                           int64_t t0, int64_t t1)
+{
+  if (t1 > m->nt) t1 = m->nt;
+  for (int64_t t = t0; t < t1; ++t)
     s[t] = element_error(m, ip_order, nc, ip, rec, t);
   return 0;
 }

@@ -106,6 +106,10 @@ int om_target_size_rec(const om_mesh* m, int ip_order, int nc,
  * Element error s_e for every tet, given the recovered field. */
 int om_element_error(const om_mesh* m, int ip_order, int nc, const double* ip,
                      const double* rec, double* s);
+/* The same for tets [t0, t1) only (bounded CPU timing samples). */
+int om_element_error_range(const om_mesh* m, int ip_order, int nc,
+                           const double* ip, const double* rec, double* s,
+                           int64_t t0, int64_t t1);
 /* apf::measure of the six edges, max per tet (getCurrentSize). */
 int om_current_size(const om_mesh* m, double* h);
 
