@@ -1,0 +1,120 @@
+"""CPU-side checks of the product library: it loads, it exports every
+symbol include/sprgpu.h declares, the host-only entry points behave, and the
+box generator numbers entities as apf::makeMdsBox does
+(mds/apfBox.cc:178-336)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from core_amd import _lib, mesh, workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "sprgpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sprg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_header():
+    L = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _lib.SIGNATURES, n
+    assert L.sprg_version().decode().startswith("sprgpu")
+
+
+def test_error_path_without_device():
+    # host-only validation runs before any device call
+    L = _lib.lib()
+    assert L.sprg_box_counts(0, None, None, None) == _lib.SPRG_EINVAL
+    assert b"n must be" in L.sprg_last_error()
+    assert L.sprg_mesh_create(None, 1, 1, 0, None, None, None, 1, None) == _lib.SPRG_EINVAL
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8])
+def test_box_counts(n):
+    nv, nt, ne = mesh.box_counts(n)
+    assert nv == (n + 1) ** 3 and nt == 6 * n ** 3
+    assert ne == 3 * n * (n + 1) ** 2 + 3 * n * n * (n + 1) + n ** 3
+    h = mesh.box(n, order=2)
+    # every tet edge is the edge of its two vertices
+    ev = h.ev
+    for k in range(6):
+        a, b = mesh.TET_EDGE_VERTS[k]
+        e = h.te[:, k]
+        assert np.all(np.sort(ev[e], 1) == np.sort(h.tv[:, [a, b]], 1))
+    # all edges used, each pair of vertices once
+    assert len(np.unique(h.te)) == ne
+    assert len({tuple(sorted(p)) for p in ev.tolist()}) == ne
+
+
+def test_box_numbering_matches_makeMdsBox():
+    n = 3
+    h = mesh.box(n, order=2)
+    g = n + 1
+    # vertices: grid order x fastest, at (1/n) * index
+    i = np.arange(g ** 3)
+    coords = np.stack([i % g, (i // g) % g, i // (g * g)], 1) * (1.0 / n)
+    assert np.array_equal(h.xyz[: h.nv], coords)
+    # tets: Kuhn table per hex in grid order
+    kuhn = [[0, 1, 2, 6], [0, 2, 3, 6], [0, 3, 7, 6], [0, 7, 4, 6], [0, 4, 5, 6], [0, 5, 1, 6]]
+    off = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    hexv = lambda x, y, z: [(x + a) + g * ((y + b) + g * (z + c)) for a, b, c in off]
+    for hidx, (x, y, z) in enumerate([(x, y, z) for z in range(n) for y in range(n) for x in range(n)]):
+        rv = hexv(x, y, z)
+        for k in range(6):
+            assert list(h.tv[6 * hidx + k]) == [rv[j] for j in kuhn[k]]
+    # edges: axis edges first (per vertex, x y z), then face diagonals,
+    # then body diagonals
+    nax = 3 * n * g * g
+    nfd = 3 * n * n * g
+    d = np.abs(h.xyz[h.ev[:, 0]] - h.xyz[h.ev[:, 1]]) * n
+    nnz = (d > 0.5).sum(1)
+    assert np.all(nnz[:nax] == 1) and np.all(nnz[nax:nax + nfd] == 2) and np.all(nnz[nax + nfd:] == 3)
+    assert list(h.ev[0]) == [0, 1] and list(h.ev[1]) == [0, g] and list(h.ev[2]) == [0, g * g]
+    # positive Kuhn orientation
+    X = h.xyz[h.tv]
+    assert np.all(np.linalg.det(X[:, 1:] - X[:, :1]) > 0)
+    # Lagrange-2 edge nodes at (0 + x0*0.5) + x1*0.5
+    mid = h.xyz[h.ev[:, 0]] * 0.5 + h.xyz[h.ev[:, 1]] * 0.5
+    assert np.array_equal(h.xyz[h.nv:], mid)
+
+
+def test_box_jitter_and_grading():
+    n = 6
+    h0 = mesh.box(n, 2)
+    h = mesh.box(n, 2, jitter=0.3, seed=workloads.C3, grade=True)
+    g = n + 1
+    i = np.arange(h.nv)
+    c = np.stack([i % g, (i // g) % g, i // (g * g)], 1)
+    boundary = ((c == 0) | (c == n)).any(1)
+    base = h0.xyz[: h.nv].copy()
+    base[:, 0] = base[:, 0] ** 2
+    V = h.xyz[: h.nv]
+    assert np.array_equal(V[boundary], base[boundary])
+    disp = np.abs(V[~boundary, 1:] - h0.xyz[: h.nv][~boundary, 1:])
+    assert disp.max() <= 0.3 / n + 1e-15 and disp.max() > 0.1 / n
+    X = h.xyz[h.tv]
+    assert np.all(np.linalg.det(X[:, 1:] - X[:, :1]) > 0)
+    # deterministic
+    h2 = mesh.box(n, 2, jitter=0.3, seed=workloads.C3, grade=True)
+    assert np.array_equal(h.xyz, h2.xyz)
+
+
+def test_splitmix_reference_values():
+    # splitmix64 of seed 0: the published first outputs
+    r = workloads.SplitMix64(0)
+    assert r.next() == 0xE220A8397B1DCDAF
+    assert r.next() == 0x6E789E6AA1B965F4
+
+
+def test_multi_bump_sizes():
+    ns = workloads.multi_bump_sizes()
+    assert len(ns) == 256 and min(ns) >= 6 and max(ns) <= 40
