@@ -79,6 +79,25 @@ struct FitArgs {
   Lists L;
 };
 
+// Appends v to list[*counter] with one atomic per wave and target list.
+// Every lane of the wave calls it (convergent), so the ballot sees the whole
+// wave.
+__device__ __forceinline__ void wave_append(int target, int my_target, int32_t* count,
+                                            int32_t* list, int32_t v)
+{
+  const uint64_t mask = __ballot(my_target == target);
+  if (!mask) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(mask));
+  base = __shfl(base, leader);
+  if (my_target == target) {
+    const uint64_t below = mask & ((1ull << lane) - 1);
+    list[base + __popcll(below)] = v;
+  }
+}
+
 // Launchers (return 0 or an SPRG_E* code).
 int launch_csr(const MeshView& m, int dim, int32_t* off, int32_t* idx,
                int32_t* tmp, hipStream_t s);

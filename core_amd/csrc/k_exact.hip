@@ -130,23 +130,6 @@ __device__ __forceinline__ void patch_set(const FitArgs& a, int64_t p,
   }
 }
 
-// Appends v to list[*counter] with one atomic per wave and target list.
-__device__ __forceinline__ void wave_append(int target, int my_target, int32_t* count,
-                                            int32_t* list, int32_t v)
-{
-  const uint64_t mask = __ballot(my_target == target);
-  if (!mask) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(count, __popcll(mask));
-  base = __shfl(base, leader);
-  if (my_target == target) {
-    const uint64_t below = mask & ((1ull << lane) - 1);
-    list[base + __popcll(below)] = v;
-  }
-}
-
 // Sorts patches into row-count buckets; patches with fewer rows than terms
 // go straight to the expansion queue (hasEnoughPoints, :355-363).  With
 // a.fast set, every regular patch goes to list 0.

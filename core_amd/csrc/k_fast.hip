@@ -31,23 +31,27 @@
 #include "device.h"
 #include "spr_math.h"
 
+#include <algorithm>
+
 namespace sprg {
 namespace {
 
 constexpr int NQ_MAX = 4;  // points per element the fast kernels handle
 
+template <bool P2MAP>
 struct Geo {
-  double x[10][3];
+  double x[P2MAP ? 10 : 4][3];
 };
 
-__device__ __forceinline__ void load_geo(const MeshView& m, int32_t t, bool p2map, Geo& g)
+template <bool P2MAP>
+__device__ __forceinline__ void load_geo(const MeshView& m, int32_t t, bool p2map, Geo<P2MAP>& g)
 {
   const int64_t tt = t;
   for (int a = 0; a < 4; ++a) {
     const double* p = m.xyz + 3 * (int64_t)m.tv[4 * tt + a];
     g.x[a][0] = p[0]; g.x[a][1] = p[1]; g.x[a][2] = p[2];
   }
-  if (p2map)
+  if (P2MAP)
     for (int e = 0; e < 6; ++e) {
       const double* p = m.xyz + 3 * (m.nv + (int64_t)m.te[6 * tt + e]);
       g.x[4 + e][0] = p[0]; g.x[4 + e][1] = p[1]; g.x[4 + e][2] = p[2];
@@ -55,13 +59,14 @@ __device__ __forceinline__ void load_geo(const MeshView& m, int32_t t, bool p2ma
 }
 
 // Sample point l of the mesh-order rule, minus the node position.
-__device__ __forceinline__ void sample_d(const MeshView& m, const Geo& g, bool p2map,
+template <bool P2MAP>
+__device__ __forceinline__ void sample_d(const MeshView& m, const Geo<P2MAP>& g, bool p2map,
                                          int l, const double* xn, double* d)
 {
   double xi[3];
   rule_point(m.order, l, xi);
   double x[3];
-  if (p2map) {
+  if constexpr (P2MAP) {
     double N[10];
     tet_values(2, xi, N);
     for (int j = 0; j < 3; ++j) {
@@ -123,95 +128,114 @@ __device__ __forceinline__ void chol_solve(const T* L, const T* rhs, T* z)
 
 enum { SRC_REGULAR = 0, SRC_EXPANDED = 1, SRC_FALLBACK = 2 };
 
-template <class T, int NT, int NC, int NREF>
+enum { ACT_SOLVE = 0, ACT_EXACT = 1, ACT_FALLBACK = 2, ACT_ILLCOND = 3, ACT_NONE = 4 };
+
+// src SRC_REGULAR walks every patch id (no list); the others walk a list.
+template <class T, int NT, int NC, int NREF, bool P2MAP>
 __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank, float tau_acc)
 {
   const MeshView& m = a.m;
   const int q = a.q;
-  const bool p2map = m.order == 2 && !m.straight;
-  const int32_t* list = src == SRC_REGULAR ? a.L.items[0]
-                        : src == SRC_EXPANDED ? a.L.xitems[0] : a.L.fbq;
-  const int count = src == SRC_REGULAR ? a.L.count[0]
-                    : src == SRC_EXPANDED ? a.L.count[NBUCKET] : a.L.count[2 * NBUCKET + 1];
+  constexpr bool p2map = P2MAP;
+  const int32_t* list = src == SRC_EXPANDED ? a.L.xitems[0] : a.L.fbq;
+  const int64_t count = src == SRC_REGULAR ? a.npatch
+                        : src == SRC_EXPANDED ? a.L.count[NBUCKET] : a.L.count[2 * NBUCKET + 1];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < count; it += stride) {
-    const int32_t entry = list[it];
-    bool expanded = src == SRC_EXPANDED || (src == SRC_FALLBACK && entry < 0);
+  // The trip count is the same for every lane of a block, so the list
+  // appends below run on whole waves; lanes past the end carry ACT_NONE.
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < count; base += stride) {
+    const int64_t it = base + threadIdx.x;
+    const bool valid = it < count;
+    const int32_t entry = !valid ? 0 : src == SRC_REGULAR ? (int32_t)it : list[it];
+    const bool expanded = src == SRC_EXPANDED || (src == SRC_FALLBACK && entry < 0);
     const int32_t id = entry & 0x7fffffff;
-    int64_t p;
-    const int32_t* set;
-    int ns;
-    if (expanded) {
-      p = a.L.x_patch[id];
-      set = a.L.x_set + a.L.x_off[id];
-      ns = a.L.x_len[id];
-    } else {
-      p = id;
-      if (p < m.nv) {
-        set = m.v2t + m.v2t_off[p];
-        ns = m.v2t_off[p + 1] - m.v2t_off[p];
+    int64_t p = 0;
+    const int32_t* set = nullptr;
+    int ns = 0;
+    if (valid) {
+      if (expanded) {
+        p = a.L.x_patch[id];
+        set = a.L.x_set + a.L.x_off[id];
+        ns = a.L.x_len[id];
       } else {
-        set = m.e2t + m.e2t_off[p - m.nv];
-        ns = m.e2t_off[p - m.nv + 1] - m.e2t_off[p - m.nv];
+        p = id;
+        if (p < m.nv) {
+          set = m.v2t + m.v2t_off[p];
+          ns = m.v2t_off[p + 1] - m.v2t_off[p];
+        } else {
+          set = m.e2t + m.e2t_off[p - m.nv];
+          ns = m.e2t_off[p - m.nv + 1] - m.e2t_off[p - m.nv];
+        }
       }
     }
-    const double xn[3] = {m.xyz[3 * p], m.xyz[3 * p + 1], m.xyz[3 * p + 2]};
-    // pass 1: Gram matrix, packed lower triangle
+    // fewer samples than terms (hasEnoughPoints): straight to expansion
+    const bool short_set = valid && src == SRC_REGULAR && ns * q < NT;
+    const bool work = valid && !short_set;
     constexpr int NG = NT * (NT + 1) / 2;
     T G[NG];
 #pragma unroll
     for (int k = 0; k < NG; ++k) G[k] = 0;
-    for (int s = 0; s < ns; ++s) {
-      Geo g;
-      load_geo(m, set[s], p2map, g);
-      for (int l = 0; l < q; ++l) {
-        double d[3];
-        sample_d(m, g, p2map, l, xn, d);
-        T f[NT];
-        basis<T, NT>(d, f);
-#pragma unroll
-        for (int i = 0; i < NT; ++i)
-#pragma unroll
-          for (int j = 0; j <= i; ++j) G[tri(i, j)] = fma(f[i], f[j], G[tri(i, j)]);
-      }
-    }
-    // Cholesky in place, with the rank certificate and the conditioning
     bool rank_ok = true;
     float min_ratio = 1.0f;
+    double xn[3] = {0, 0, 0};
+    if (work) {
+      xn[0] = m.xyz[3 * p];
+      xn[1] = m.xyz[3 * p + 1];
+      xn[2] = m.xyz[3 * p + 2];
+      // pass 1: Gram matrix, packed lower triangle
+      for (int s = 0; s < ns; ++s) {
+        Geo<P2MAP> g;
+        load_geo(m, set[s], p2map, g);
+        for (int l = 0; l < q; ++l) {
+          double d[3];
+          sample_d(m, g, p2map, l, xn, d);
+          T f[NT];
+          basis<T, NT>(d, f);
 #pragma unroll
-    for (int k = 0; k < NT; ++k) {
-      const T diag = G[tri(k, k)];
-      T dk = diag;
+          for (int i = 0; i < NT; ++i)
 #pragma unroll
-      for (int j = 0; j < k; ++j) dk = fma(-G[tri(k, j)], G[tri(k, j)], dk);
-      const float ratio = (diag > 0 && dk > 0) ? (float)(dk / diag) : 0.0f;
-      min_ratio = fminf(min_ratio, ratio);
-      const T lkk = dk > 0 ? sqrt(dk) : (T)0;
-      if (!(ratio >= tau_rank) || !((double)lkk >= 1.05e-10)) rank_ok = false;
-      G[tri(k, k)] = lkk > 0 ? lkk : (T)1;
+            for (int j = 0; j <= i; ++j) G[tri(i, j)] = fma(f[i], f[j], G[tri(i, j)]);
+        }
+      }
+      // Cholesky in place, with the rank certificate and the conditioning
 #pragma unroll
-      for (int i = k + 1; i < NT; ++i) {
-        T s = G[tri(i, k)];
+      for (int k = 0; k < NT; ++k) {
+        const T diag = G[tri(k, k)];
+        T dk = diag;
 #pragma unroll
-        for (int j = 0; j < k; ++j) s = fma(-G[tri(i, j)], G[tri(k, j)], s);
-        G[tri(i, k)] = s / G[tri(k, k)];
+        for (int j = 0; j < k; ++j) dk = fma(-G[tri(k, j)], G[tri(k, j)], dk);
+        const float ratio = (diag > 0 && dk > 0) ? (float)(dk / diag) : 0.0f;
+        min_ratio = fminf(min_ratio, ratio);
+        const T lkk = dk > 0 ? sqrt(dk) : (T)0;
+        if (!(ratio >= tau_rank) || !((double)lkk >= 1.05e-10)) rank_ok = false;
+        G[tri(k, k)] = lkk > 0 ? lkk : (T)1;
+#pragma unroll
+        for (int i = k + 1; i < NT; ++i) {
+          T s = G[tri(i, k)];
+#pragma unroll
+          for (int j = 0; j < k; ++j) s = fma(-G[tri(i, j)], G[tri(k, j)], s);
+          G[tri(i, k)] = s / G[tri(k, k)];
+        }
       }
     }
-    if (src == SRC_REGULAR && !rank_ok) {
-      // not certified: the exact path decides (test the initial set first)
-      int k = atomicAdd(a.L.count + 2 * NBUCKET, 1);
-      a.L.xq[k] = (int32_t)(p | 0x80000000u);
-      continue;
-    }
-    if (!(min_ratio >= tau_acc)) {
-      if (src == SRC_FALLBACK) {
-        atomicOr(a.L.status, ST_ILLCOND);
-      } else {
-        int k = atomicAdd(a.L.count + 2 * NBUCKET + 1, 1);
-        a.L.fbq[k] = expanded ? (int32_t)(id | 0x80000000u) : (int32_t)p;
-      }
-      continue;
-    }
+    // short sets and uncertified ranks: the exact path decides (bit 31: test
+    // the initial set before growing it); poorly conditioned: the FP64
+    // fallback
+    int act = ACT_NONE;
+    if (short_set)
+      act = ACT_EXACT;
+    else if (work && src == SRC_REGULAR && !rank_ok)
+      act = ACT_EXACT;
+    else if (work && !(min_ratio >= tau_acc))
+      act = src == SRC_FALLBACK ? ACT_ILLCOND : ACT_FALLBACK;
+    else if (work)
+      act = ACT_SOLVE;
+    wave_append(ACT_EXACT, act, a.L.count + 2 * NBUCKET, a.L.xq,
+                short_set ? (int32_t)p : (int32_t)(p | 0x80000000u));
+    wave_append(ACT_FALLBACK, act, a.L.count + 2 * NBUCKET + 1, a.L.fbq,
+                expanded ? (int32_t)(id | 0x80000000u) : (int32_t)p);
+    if (act == ACT_ILLCOND) atomicOr(a.L.status, ST_ILLCOND);
+    if (act != ACT_SOLVE) continue;
     // z = G^-1 e_0, refined against the FP64 residual
     T e0[NT], zt[NT];
 #pragma unroll
@@ -225,7 +249,7 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
 #pragma unroll
       for (int k = 0; k < NT; ++k) u[k] = 0;
       for (int s = 0; s < ns; ++s) {
-        Geo g;
+        Geo<P2MAP> g;
         load_geo(m, set[s], p2map, g);
         for (int l = 0; l < q; ++l) {
           double d[3], f[NT];
@@ -253,7 +277,7 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
       for (int c = 0; c < NC; ++c) { b0[c] = bp[c]; v[c] = 0; }
     }
     for (int s = 0; s < ns; ++s) {
-      Geo g;
+      Geo<P2MAP> g;
       const int32_t t = set[s];
       load_geo(m, t, p2map, g);
       for (int l = 0; l < q; ++l) {
@@ -274,18 +298,28 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
   }
 }
 
-template <class T, int NT, int NREF>
-int launch_nc(const FitArgs& a, int src, float tau_rank, float tau_acc, hipStream_t s)
+template <class T, int NT, int NREF, bool P2>
+int launch_nc2(const FitArgs& a, int src, float tau_rank, float tau_acc, hipStream_t s)
 {
-  const unsigned blocks = 4096, threads = 256;
+  const unsigned threads = 256;
+  const int64_t work = src == SRC_REGULAR ? a.npatch : a.npatch / 8 + 1;
+  const unsigned blocks = (unsigned)std::min<int64_t>(8192, (work + threads - 1) / threads);
   switch (a.nc) {
-    case 1: k_fast<T, NT, 1, NREF><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
-    case 3: k_fast<T, NT, 3, NREF><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
-    case 9: k_fast<T, NT, 9, NREF><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
-    case 27: k_fast<T, NT, 27, NREF><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
+    case 1: k_fast<T, NT, 1, NREF, P2><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
+    case 3: k_fast<T, NT, 3, NREF, P2><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
+    case 9: k_fast<T, NT, 9, NREF, P2><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
+    case 27: k_fast<T, NT, 27, NREF, P2><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
     default: return sprg_fail(SPRG_EUNSUPPORTED, "fast recovery: %d components", a.nc);
   }
   return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "fast fit launch failed");
+}
+
+template <class T, int NT, int NREF>
+int launch_nc(const FitArgs& a, int src, float tau_rank, float tau_acc, hipStream_t s)
+{
+  if (a.m.order == 2 && !a.m.straight)
+    return launch_nc2<T, NT, NREF, true>(a, src, tau_rank, tau_acc, s);
+  return launch_nc2<T, NT, NREF, false>(a, src, tau_rank, tau_acc, s);
 }
 
 template <class T, int NREF>

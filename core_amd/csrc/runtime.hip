@@ -412,10 +412,10 @@ int run_recover(sprg_mesh* m, int q, int nc, const double* ip, double* out, int 
     HIP_OK(hipMemsetAsync(a.L.count, 0, sizeof(int32_t) * (2 * NBUCKET + 4), s), "memset");
     m->phase_last = 0;
     m->mark(0);
-    if ((rc = launch_classify(a, nterms, s))) return rc;
     if (fast) {
       if ((rc = launch_fast_fit(a, nterms, form, 0, s))) return rc;
     } else {
+      if ((rc = launch_classify(a, nterms, s))) return rc;
       if ((rc = launch_exact_fit(a, nterms, false, s))) return rc;
     }
     m->mark(1);
