@@ -10,8 +10,13 @@ points of the order-2 tet rule; one step = spr::getSPRSizeField(eps, 0.05)
 size factor, vertex average) with the IP field resident in HBM.
 
 Prints one JSON line.  value = recovered patches (V + E) per second of the
-whole size-field step; roofline = the recovery phase's algorithmic HBM bytes
-over its GPU time (HIP events inside the library on its stream);
+whole size-field step in the chosen form (default fp64f: FP64 normal
+equations, the fastest form that meets 1e-6 relative on the size field);
+"forms" reports the other two forms on the same mesh (FP32 + FP64 refinement,
+and the exact FP64 form that reproduces the reference bit for bit) with
+their accuracy against the exact form; roofline = the dominant kernel (the
+patch fit over every patch, one launch) with the recovery's algorithmic HBM
+bytes over its GPU time (HIP events the library records on its stream);
 cpu_baseline = the FP64 CPU oracle on a bounded sample of the same workload.
 """
 import argparse
@@ -90,11 +95,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--form", default="fp32r", choices=["fp64", "fp32r", "fp64f"])
+    ap.add_argument("--form", default="fp64f", choices=["fp64", "fp32r", "fp64f"])
     ap.add_argument("--n", type=int, default=120)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--exact-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -117,31 +123,34 @@ def main():
     torch.cuda.synchronize()
     log("uploaded (%.1fs)" % (time.time() - t0))
 
-    def step():
-        s = spr.getSPRSizeField(eps, 0.05, form=args.form)
+    def step(form):
+        s = spr.getSPRSizeField(eps, 0.05, form=form)
         m.destroyField(s)
         return m.phase_times()
 
-    for i in range(args.warmup):
-        step()
-        log("warmup %d done" % i)
-    phases = np.zeros(5)
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        phases += step()
-    torch.cuda.synchronize()
-    t_end = time.perf_counter()
-    ms_per_step = (t_end - t_start) * 1e3 / args.steps
-    phases /= args.steps
-    log("timed %d steps: %.3f ms/step; phases ms %s" % (args.steps, ms_per_step,
-                                                       np.array2string(phases, precision=3)))
+    def timed_steps(form, warmup, steps):
+        for i in range(warmup):
+            step(form)
+        phases = np.zeros(5)
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for i in range(steps):
+            phases += step(form)
+        torch.cuda.synchronize()
+        t_end = time.perf_counter()
+        return (t_end - t_start) * 1e3 / steps, phases / steps
+
+    ms_per_step, phases = timed_steps(args.form, args.warmup, args.steps)
+    log("%s: timed %d steps: %.3f ms/step; phases ms %s" % (
+        args.form, args.steps, ms_per_step, np.array2string(phases, precision=3)))
     T, V, E = host.nt, host.nv, host.ne
     patches = V + E
     value = patches / (ms_per_step / 1e3)
     ab = algorithmic_bytes(T, V, E)
-    rec_ms = float(phases[0] + phases[1] + phases[2])
-    achieved = ab["recover"] / (rec_ms / 1e3) / 1e9
+    # dominant kernel: the fit of every patch (one launch; phase 0).  In the
+    # exact form phase 0 is the classify pass plus the bucketed fits.
+    fit_ms = float(phases[0])
+    achieved = ab["recover"] / (fit_ms / 1e3) / 1e9
     out = {
         "metric": "SPR patches/s & size-field ms @10.4M tets: FP32+refine vs FP64, rel err<=1e-6",
         "value": value,
@@ -162,33 +171,46 @@ def main():
                       "element": phases[3], "factor_average": phases[4]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "recovery (fit + expand + expanded fit), %d algorithmic bytes"
-                               % ab["recover"]},
+                     "kernel": ("k_fast regular pass" if args.form != "fp64" else "exact fit")
+                               + " (all %d patches), %d algorithmic bytes per launch, %.3f ms"
+                               % (patches, ab["recover"], fit_ms)},
     }
-    if args.form != "fp64" and not args.no_compare:
-        # accuracy against the FP64 form (bit-identical to the FP64 oracle)
-        s_fast = spr.getSPRSizeField(eps, 0.05, form=args.form)
-        a_fast = s_fast.data[:, 0].clone()
-        m.destroyField(s_fast)
+    if not args.no_compare:
+        # the other forms on the same mesh; accuracy against the exact FP64
+        # form (bit-identical to the FP64 oracle)
         s64 = spr.getSPRSizeField(eps, 0.05, form="fp64")
-        fp64_ms = float(m.phase_times().sum())
         a64 = s64.data[:, 0].clone()
         m.destroyField(s64)
-        rel = (a_fast / a64 - 1).abs()
-        r_fast = spr.recoverField(eps, form=args.form)
-        rf = r_fast.data.clone()
-        m.destroyField(r_fast)
         r64 = spr.recoverField(eps, form="fp64")
-        scale = r64.data.abs().amax(0)
-        keep = scale > 0
-        rr = ((rf - r64.data).abs()[:, keep] / scale[keep])
+        d64 = r64.data.clone()
         m.destroyField(r64)
-        out["accuracy_vs_fp64"] = {
-            "size_max_rel": float(rel.max()), "size_rms_rel": float(rel.pow(2).mean().sqrt()),
-            "recovered_max_rel": float(rr.max()), "recovered_rms_rel": float(rr.pow(2).mean().sqrt()),
-            "fp64_form_gpu_ms": fp64_ms,
-            "speedup_vs_fp64_form": fp64_ms / float(phases.sum())}
-        log("accuracy vs fp64: %s" % out["accuracy_vs_fp64"])
+        scale = d64.abs().amax(0)
+        keep = scale > 0
+        forms = {}
+        for form in ["fp64f", "fp32r", "fp64"]:
+            if form == args.form:
+                ms, ph = ms_per_step, phases
+            elif form == "fp64":
+                ms, ph = timed_steps(form, 1, max(1, args.exact_steps))
+            else:
+                ms, ph = timed_steps(form, 1, args.steps)
+            rec = {"ms_per_step": ms, "patches_per_s": patches / (ms / 1e3),
+                   "phases_ms": [float(x) for x in ph]}
+            if form != "fp64":
+                sf = spr.getSPRSizeField(eps, 0.05, form=form)
+                rel = (sf.data[:, 0] / a64 - 1).abs()
+                m.destroyField(sf)
+                rf = spr.recoverField(eps, form=form)
+                rr = (rf.data - d64).abs()[:, keep] / scale[keep]
+                m.destroyField(rf)
+                rec.update(size_max_rel=float(rel.max()), size_rms_rel=float(rel.pow(2).mean().sqrt()),
+                           recovered_max_rel=float(rr.max()),
+                           recovered_rms_rel=float(rr.pow(2).mean().sqrt()))
+            forms[form] = rec
+            log("form %s: %s" % (form, rec))
+        for form in ("fp64f", "fp32r"):
+            forms[form]["speedup_vs_fp64_exact"] = forms["fp64"]["ms_per_step"] / forms[form]["ms_per_step"]
+        out["forms"] = forms
     if not args.no_cpu:
         log("cpu baseline (budget %.0f s)" % args.cpu_budget)
         out["cpu_baseline"] = cpu_baseline(host, ip, args.cpu_budget)
