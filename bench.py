@@ -46,38 +46,83 @@ def algorithmic_bytes(T, V, E, q=4, c=9, order=2):
     return {"recover": recover, "size": size, "average": average}
 
 
-def cpu_baseline(host, ip, budget_s):
-    """Time the FP64 oracle on a sample of patches and tets of the workload
-    and scale to the whole step (serial, one core)."""
-    import numpy as np
+def copy_probe(torch, nbytes=1 << 30, reps=5):
+    """Device-to-device copy of nbytes (a plain read+write kernel): the
+    measured HBM rate (2 * nbytes per copy) and the reference for scaling
+    rocprofv3 FETCH_SIZE / WRITE_SIZE readings."""
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del src, dst
+    return {"bytes_per_copy": 2 * nbytes, "best_ms": best, "gbs": 2 * nbytes / (best / 1e3) / 1e9}
+
+
+_CPU = {}
+
+
+def _oracle_timings(job):
+    """Worker: net oracle time of one chunk of vertex patches, edge patches
+    and tets (the upward-adjacency rebuild each oracle recover call starts
+    with, timed on an empty range, is mesh setup and is taken out)."""
     from oracle import oracle as O
 
-    om = O.Mesh(host.tv, host.xyz, host.order, te=host.te, ev=host.ev)
-    V, E, T = host.nv, host.ne, host.nt
-    rec = np.zeros((host.nnodes, ip.shape[2]))
+    om, ip, rec = _CPU["om"], _CPU["ip"], _CPU["rec"]
+    (v0, v1), (e0, e1), (t0, t1) = job
 
     def timed(fn, *a):
         t = time.perf_counter()
         fn(*a)
         return time.perf_counter() - t
 
-    # every oracle recover call rebuilds the mesh's upward adjacency first;
-    # that fixed cost (an empty range) is mesh setup, not per-patch work
     fixed = min(timed(O.recover, om, ip, 0, 0) for _ in range(2))
+    dv = max(timed(O.recover, om, ip, v0, v1) - fixed, 1e-9) if v1 > v0 else 0.0
+    de = max(timed(O.recover, om, ip, e0, e1) - fixed, 1e-9) if e1 > e0 else 0.0
+    dt = timed(O.element_error_range, om, 2, ip, rec, t0, t1) if t1 > t0 else 0.0
+    return dv, de, dt
+
+
+def cpu_baseline(host, ip, budget_s, workers):
+    """The FP64 oracle on a sample of patches and tets from the middle of the
+    workload, scaled to the whole step: serially on one core, then split over
+    `workers` processes (one per core, each on its own contiguous share, as a
+    partitioned run would be).  Call it before the GPU is touched: the
+    workers are forked."""
+    import multiprocessing
+
+    import numpy as np
+    from oracle import oracle as O
+
+    _CPU["om"] = O.Mesh(host.tv, host.xyz, host.order, te=host.te, ev=host.ev)
+    _CPU["ip"] = ip
+    _CPU["rec"] = np.zeros((host.nnodes, ip.shape[2]))
+    V, E, T = host.nv, host.ne, host.nt
     mid_v, mid_e, mid_t = V // 2, V + E // 2, T // 2
 
-    def costs(nv_s, ne_s, nt_s):
-        tv = max(timed(O.recover, om, ip, mid_v, mid_v + nv_s) - fixed, 1e-9) / nv_s
-        te = max(timed(O.recover, om, ip, mid_e, mid_e + ne_s) - fixed, 1e-9) / ne_s
-        tt = timed(O.element_error_range, om, 2, ip, rec, mid_t, mid_t + nt_s) / nt_s
-        return tv, te, tt
+    def job(nv_s, ne_s, nt_s, k=0, parts=1):
+        def share(base, n):
+            a = base + n * k // parts
+            return a, base + n * (k + 1) // parts
+        return share(mid_v, nv_s), share(mid_e, ne_s), share(mid_t, nt_s)
 
-    tv, te, tt = costs(50, 300, 5000)  # probe, then size the sample to the budget
-    frac = budget_s / (tv * V + te * E + tt * T)
+    dv, de, dt = _oracle_timings(job(50, 300, 5000))  # probe, then size the sample
+    tv, te, tt = dv / 50, de / 300, dt / 5000
+    frac = min(0.5, budget_s / (tv * V + te * E + tt * T))
     nv_s, ne_s, nt_s = max(50, int(V * frac)), max(300, int(E * frac)), max(5000, int(T * frac))
-    tv, te, tt = costs(nv_s, ne_s, nt_s)
+    nv_s, ne_s, nt_s = min(nv_s, V - mid_v), min(ne_s, E // 2), min(nt_s, T - mid_t)
+    dv, de, dt = _oracle_timings(job(nv_s, ne_s, nt_s))
+    tv, te, tt = dv / nv_s, de / ne_s, dt / nt_s
     step_s = tv * V + te * E + tt * T
-    return {
+    out = {
         "value": (V + E) / step_s,
         "unit": "patches/s",
         "cores": 1,
@@ -88,19 +133,33 @@ def cpu_baseline(host, ip, budget_s):
                    "(est. full CPU step %.0f s; vertex patch %.3f ms, edge patch %.3f ms, "
                    "tet %.2f us)" % (nv_s, ne_s, nt_s, step_s, tv * 1e3, te * 1e3, tt * 1e6)),
     }
+    if workers > 1:
+        with multiprocessing.get_context("fork").Pool(workers) as pool:
+            res = pool.map(_oracle_timings, [job(nv_s, ne_s, nt_s, k, workers) for k in range(workers)])
+        slowest = max(sum(r) for r in res)
+        par_s = slowest * (V / nv_s)
+        out["parallel"] = {
+            "value": (V + E) / par_s, "unit": "patches/s", "cores": workers, "kind": "port",
+            "sample": ("the same sample split into %d contiguous shares, one forked process per "
+                       "core; slowest share %.2f s scaled to the whole step (est. %.1f s)"
+                       % (workers, slowest, par_s))}
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", default="fp64f", choices=["fp64", "fp32r", "fp64f"])
     ap.add_argument("--n", type=int, default=120)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--exact-steps", type=int, default=2)
+    ap.add_argument("--copy-probe-only", action="store_true",
+                    help="run the 1 GiB device copy probe and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -111,6 +170,10 @@ def main():
     import torch
     from core_amd import spr, workloads
 
+    if args.copy_probe_only:
+        print(json.dumps({"copy_probe": copy_probe(torch)}), flush=True)
+        return 0
+
     t0 = time.time()
     host = workloads.strain_mesh(args.n)
     log("mesh n=%d: %d tets, %d vertices, %d edges (%.1fs)" % (args.n, host.nt, host.nv, host.ne,
@@ -118,6 +181,14 @@ def main():
     ip = workloads.strain_ip(host, progress=lambda d, n: log("  strain IP field %d/%d tets" % (d, n))
                              if d % (1 << 23) < (1 << 20) or d == n else None)
     log("strain IP field %s (%.1fs)" % (ip.shape, time.time() - t0))
+    cpu = None
+    if not args.no_cpu:
+        # before the GPU is initialised: the parallel baseline forks
+        log("cpu baseline (budget %.0f s, %d workers)" % (args.cpu_budget, args.cpu_workers))
+        cpu = cpu_baseline(host, ip, args.cpu_budget, args.cpu_workers)
+        log("cpu baseline: %s" % cpu)
+    probe = copy_probe(torch)
+    log("copy probe: %s" % probe)
     m = spr.Mesh(host)
     eps = m.createIPField("eps", 9, 2, ip)
     torch.cuda.synchronize()
@@ -128,16 +199,22 @@ def main():
         m.destroyField(s)
         return m.phase_times()
 
+    per_step = {}
+
     def timed_steps(form, warmup, steps):
         for i in range(warmup):
             step(form)
         phases = np.zeros(5)
+        each = []
         torch.cuda.synchronize()
         t_start = time.perf_counter()
         for i in range(steps):
-            phases += step(form)
+            t = time.perf_counter()
+            phases += step(form)  # returns after the call's work is complete
+            each.append((time.perf_counter() - t) * 1e3)
         torch.cuda.synchronize()
         t_end = time.perf_counter()
+        per_step[form] = each
         return (t_end - t_start) * 1e3 / steps, phases / steps
 
     ms_per_step, phases = timed_steps(args.form, args.warmup, args.steps)
@@ -159,6 +236,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        "ms_per_step_median": float(np.median(per_step[args.form])),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -169,8 +247,10 @@ def main():
                    "form": args.form, "tets": T, "vertices": V, "edges": E, "patches": patches},
         "phases_ms": {"fit": phases[0], "expand": phases[1], "expanded_fit": phases[2],
                       "element": phases[3], "factor_average": phases[4]},
+        "routes": m.last_stats(),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "measured_copy_gbs": probe["gbs"],
                      "kernel": ("k_fast regular pass" if args.form != "fp64" else "exact fit")
                                + " (all %d patches), %d algorithmic bytes per launch, %.3f ms"
                                % (patches, ab["recover"], fit_ms)},
@@ -194,8 +274,9 @@ def main():
                 ms, ph = timed_steps(form, 1, max(1, args.exact_steps))
             else:
                 ms, ph = timed_steps(form, 1, args.steps)
-            rec = {"ms_per_step": ms, "patches_per_s": patches / (ms / 1e3),
-                   "phases_ms": [float(x) for x in ph]}
+            rec = {"ms_per_step": ms, "ms_median": float(np.median(per_step[form])),
+                   "patches_per_s": patches / (ms / 1e3),
+                   "phases_ms": [float(x) for x in ph], "routes": m.last_stats()}
             if form != "fp64":
                 sf = spr.getSPRSizeField(eps, 0.05, form=form)
                 rel = (sf.data[:, 0] / a64 - 1).abs()
@@ -211,9 +292,8 @@ def main():
         for form in ("fp64f", "fp32r"):
             forms[form]["speedup_vs_fp64_exact"] = forms["fp64"]["ms_per_step"] / forms[form]["ms_per_step"]
         out["forms"] = forms
-    if not args.no_cpu:
-        log("cpu baseline (budget %.0f s)" % args.cpu_budget)
-        out["cpu_baseline"] = cpu_baseline(host, ip, args.cpu_budget)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     return 0
 

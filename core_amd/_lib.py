@@ -44,12 +44,17 @@ SIGNATURES = {
     "sprg_mesh_counts": (_c_int, [_vp, _vp]),
     "sprg_mesh_adjacency": (_c_int, [_vp, _c_int, _pp, _pp]),
     "sprg_mesh_phase_times": (_c_int, [_vp, _vp, _c_int]),
+    "sprg_mesh_last_stats": (_c_int, [_vp, _vp, _c_int]),
     "sprg_grad_ip": (_c_int, [_vp, _c_int, _c_int, _vp, _c_int, _vp]),
     "sprg_patch_sets": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp]),
     "sprg_recover": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int]),
     "sprg_spr_size": (_c_int, [_vp, _c_int, _c_int, _vp, _c_dbl, _vp, _c_int, _vp]),
     "sprg_target_spr_size": (_c_int, [_vp, _c_int, _c_int, _vp, _c_dbl, _c_dbl, _c_dbl, _vp,
                                       _c_int, _vp]),
+    "sprg_batch_create": (_c_int, [_vp, _c_int, _vp, _pp]),
+    "sprg_batch_destroy": (_c_int, [_vp]),
+    "sprg_batch_size": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_dbl, _c_dbl, _c_dbl, _vp, _c_int,
+                                 _vp]),
     "sprg_box_counts": (_c_int, [_c_int, _vp, _vp, _vp]),
     "sprg_box_build": (_c_int, [_c_int, _c_int, _c_dbl, ctypes.c_uint64, _c_int, _vp, _vp, _vp,
                                 _vp]),

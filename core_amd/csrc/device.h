@@ -102,8 +102,11 @@ __device__ __forceinline__ void wave_append(int target, int my_target, int32_t* 
 int launch_csr(const MeshView& m, int dim, int32_t* off, int32_t* idx,
                int32_t* tmp, hipStream_t s);
 int launch_scan_i32(int32_t* a, int64_t n, int32_t* work, hipStream_t s);
+int launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s);
 int launch_classify(const FitArgs& a, int nterms, hipStream_t s);
-int launch_exact_fit(const FitArgs& a, int nterms, bool expanded, hipStream_t s);
+// mode 0: regular patches, 1: expanded sets, 2: the fast forms' refits
+// (items[] lists holding patch ids or, with bit 31, expanded-set ids).
+int launch_exact_fit(const FitArgs& a, int nterms, int mode, hipStream_t s);
 int launch_expand(const FitArgs& a, int nterms, hipStream_t s);
 int launch_fast_fit(const FitArgs& a, int nterms, int form, int src, hipStream_t s);
 int launch_grad(const MeshView& m, int field_order, int nc_in,

@@ -159,9 +159,15 @@ __global__ __launch_bounds__(256) void k_classify(FitArgs a, int nterms)
   wave_append(NBUCKET, target, a.L.count + 2 * NBUCKET, a.L.xq, (int32_t)p);
 }
 
-template <int NT, int MMAX, int LANES, bool EXP>
+// MODE: FIT_REGULAR walks items[bucket] (patch ids), FIT_EXPANDED walks
+// xitems[bucket] (expanded-set ids), FIT_MIXED walks items[bucket] holding
+// either (bit 31 set: an expanded-set id) -- the fast forms' refits.
+enum { FIT_REGULAR = 0, FIT_EXPANDED = 1, FIT_MIXED = 2 };
+
+template <int NT, int MMAX, int LANES, int MODE>
 __global__ __launch_bounds__(64) void k_exact_fit(FitArgs a, int bucket)
 {
+  constexpr bool EXP = MODE == FIT_EXPANDED;
   constexpr int G = 64 / LANES;
   __shared__ double sR[G][MMAX * NT];
   __shared__ double sV[G][NT * MMAX];
@@ -182,13 +188,14 @@ __global__ __launch_bounds__(64) void k_exact_fit(FitArgs a, int bucket)
     int ns = 0;
     const int32_t* set = nullptr;
     if (act) {
-      int32_t it = list[item];
-      if (EXP) {
+      const int32_t entry = list[item];
+      if (EXP || (MODE == FIT_MIXED && entry < 0)) {
+        const int32_t it = entry & 0x7fffffff;
         p = a.L.x_patch[it];
         set = a.L.x_set + a.L.x_off[it];
         ns = a.L.x_len[it];
       } else {
-        p = it;
+        p = entry;
         patch_set(a, p, &set, &ns);
       }
     }
@@ -199,7 +206,7 @@ __global__ __launch_bounds__(64) void k_exact_fit(FitArgs a, int bucket)
     __syncthreads();
     bool ok = r_recurrence<NT, LANES, MMAX>(sR[g], sV[g], sDot[g], sNorm[g], mrows, gl, act);
     if (act && !ok && gl == 0) {
-      if (EXP) {
+      if (MODE != FIT_REGULAR) {  // sets whose rank was already decided
         atomicOr(a.L.status, ST_INTERNAL);
       } else {
         int k = atomicAdd(a.L.count + 2 * NBUCKET, 1);
@@ -298,6 +305,61 @@ __device__ __forceinline__ int shared_verts(const int32_t* a, const int32_t* b)
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) s += (a[i] == b[j]);
   return s;
+}
+
+// Rank screen of a candidate set, one wave: the FP64 Gram matrix of the
+// monomials centred at the patch node, summed over the rows lane by lane and
+// across the wave (butterfly, so every lane holds the same sums), then its
+// Cholesky pivots.  Centring is a unit-triangular change of the graded
+// monomial basis, so in exact arithmetic pivot k equals the reference's
+// Householder column norm R_kk (mth/mthQR.cc:31).  Returns true only when
+// every pivot clears the reference's 1e-10 by 5% and none lost more than 11
+// of the diagonal's 16 digits to cancellation; then the reference's test
+// passes too.  A false return decides nothing: the exact test runs.
+template <int NT>
+__device__ bool certify_full_rank(const MeshView& m, int q, const int32_t* S, int nrows,
+                                  int64_t p, int lane)
+{
+  constexpr int NG = NT * (NT + 1) / 2;
+  double G[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) G[k] = 0;
+  const double xn[3] = {m.xyz[3 * p], m.xyz[3 * p + 1], m.xyz[3 * p + 2]};
+  for (int i = lane; i < nrows; i += 64) {
+    double x[3], d[3], f[10];
+    sample_point(m, S[i / q], i % q, x);
+    for (int j = 0; j < 3; ++j) d[j] = x[j] - xn[j];
+    poly_terms(NT == 4 ? 1 : 2, d, f);
+    int k = 0;
+#pragma unroll
+    for (int r = 0; r < NT; ++r)
+#pragma unroll
+      for (int c = 0; c <= r; ++c, ++k) G[k] = fma(f[r], f[c], G[k]);
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+    for (int k = 0; k < NG; ++k) G[k] += __shfl_xor(G[k], off);
+  bool rank_ok = true;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    const int kk = k * (k + 1) / 2 + k;
+    const double diag = G[kk];
+    double dk = diag;
+#pragma unroll
+    for (int j = 0; j < k; ++j) dk = fma(-G[k * (k + 1) / 2 + j], G[k * (k + 1) / 2 + j], dk);
+    const double lkk = dk > 0 ? sqrt(dk) : 0.0;
+    if (!(diag > 0 && dk >= 1e-11 * diag) || !(lkk >= 1.05e-10)) rank_ok = false;
+    G[kk] = lkk > 0 ? lkk : 1.0;
+#pragma unroll
+    for (int i = k + 1; i < NT; ++i) {
+      double sacc = G[i * (i + 1) / 2 + k];
+#pragma unroll
+      for (int j = 0; j < k; ++j) sacc = fma(-G[i * (i + 1) / 2 + j], G[k * (k + 1) / 2 + j], sacc);
+      G[i * (i + 1) / 2 + k] = sacc / G[kk];
+    }
+  }
+  return rank_ok;
 }
 
 template <int NT>
@@ -401,9 +463,13 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
         const int mrows = ns * q;
         bool ok = false;
         if (mrows >= NT && mrows <= XMAXR) {
-          build_rows<NT, 64>(m, q, sCur, mrows, sR, lane);
-          __syncthreads();
-          ok = r_recurrence<NT, 64, XMAXR>(sR, sV, sDot, sNorm, mrows, lane, true);
+          // a certified full rank needs no exact test; anything else does
+          ok = certify_full_rank<NT>(m, q, sCur, mrows, p, lane);
+          if (!ok) {
+            build_rows<NT, 64>(m, q, sCur, mrows, sR, lane);
+            __syncthreads();
+            ok = r_recurrence<NT, 64, XMAXR>(sR, sV, sDot, sNorm, mrows, lane, true);
+          }
         } else if (mrows > XMAXR) {
           if (lane == 0) sOverflow = 1;
           __syncthreads();
@@ -445,7 +511,7 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
 
 inline unsigned grid_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
-template <int NT, bool EXP>
+template <int NT, int EXP>
 int fit_all(const FitArgs& a, hipStream_t s)
 {
   // Persistent grids: each block loops over its share of the bucket's
@@ -469,11 +535,13 @@ int launch_classify(const FitArgs& a, int nterms, hipStream_t s)
   return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "classify launch failed");
 }
 
-int launch_exact_fit(const FitArgs& a, int nterms, bool expanded, hipStream_t s)
+int launch_exact_fit(const FitArgs& a, int nterms, int mode, hipStream_t s)
 {
-  if (nterms == 4)
-    return expanded ? fit_all<4, true>(a, s) : fit_all<4, false>(a, s);
-  return expanded ? fit_all<10, true>(a, s) : fit_all<10, false>(a, s);
+  switch (mode) {
+    case FIT_REGULAR: return nterms == 4 ? fit_all<4, FIT_REGULAR>(a, s) : fit_all<10, FIT_REGULAR>(a, s);
+    case FIT_EXPANDED: return nterms == 4 ? fit_all<4, FIT_EXPANDED>(a, s) : fit_all<10, FIT_EXPANDED>(a, s);
+    default: return nterms == 4 ? fit_all<4, FIT_MIXED>(a, s) : fit_all<10, FIT_MIXED>(a, s);
+  }
 }
 
 int launch_expand(const FitArgs& a, int nterms, hipStream_t s)

@@ -128,7 +128,9 @@ __device__ __forceinline__ void chol_solve(const T* L, const T* rhs, T* z)
 
 enum { SRC_REGULAR = 0, SRC_EXPANDED = 1, SRC_FALLBACK = 2 };
 
-enum { ACT_SOLVE = 0, ACT_EXACT = 1, ACT_FALLBACK = 2, ACT_ILLCOND = 3, ACT_NONE = 4 };
+// ACT_EXACT: the exact path decides the rank (and expands); ACT_REFIT: the
+// exact kernel refits the certified set with the reference's arithmetic.
+enum { ACT_SOLVE = 0, ACT_EXACT = 1, ACT_FALLBACK = 2, ACT_ILLCOND = 3, ACT_NONE = 4, ACT_REFIT = 5 };
 
 // src SRC_REGULAR walks every patch id (no list); the others walk a list.
 template <class T, int NT, int NC, int NREF, bool P2MAP>
@@ -218,57 +220,82 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
         }
       }
     }
-    // short sets and uncertified ranks: the exact path decides (bit 31: test
-    // the initial set before growing it); poorly conditioned: the FP64
-    // fallback
+    // Short sets and uncertified ranks: the exact path decides (bit 31: test
+    // the initial set before growing it).  Below tau_acc the FP32 regular
+    // pass hands the patch to the FP64 fallback; an FP64 pass hands it to
+    // the exact kernel (normal equations square the condition number; the
+    // reference's Householder arithmetic reproduces its own answer), when the
+    // set fits the exact kernel's rows.
+    const int bucket = bucket_of(ns * q);
+    const int poor = (src == SRC_REGULAR && sizeof(T) == 4) ? ACT_FALLBACK
+                     : bucket < NBUCKET                     ? ACT_REFIT
+                     : src != SRC_FALLBACK                  ? ACT_FALLBACK
+                     // more rows than the exact kernel takes: the refined
+                     // FP64 solve, unless hopeless
+                     : min_ratio >= 1e-13f ? ACT_SOLVE : ACT_ILLCOND;
     int act = ACT_NONE;
     if (short_set)
       act = ACT_EXACT;
     else if (work && src == SRC_REGULAR && !rank_ok)
       act = ACT_EXACT;
     else if (work && !(min_ratio >= tau_acc))
-      act = src == SRC_FALLBACK ? ACT_ILLCOND : ACT_FALLBACK;
+      act = poor;
     else if (work)
       act = ACT_SOLVE;
+    // z = G^-1 e_0, refined against the FP64 residual; a refinement that has
+    // not settled (last correction above 1e-13 of |z|) counts as poorly
+    // conditioned
+    double z[NT];
+    if (act == ACT_SOLVE) {
+      T e0[NT], zt[NT];
+#pragma unroll
+      for (int k = 0; k < NT; ++k) e0[k] = k == 0 ? 1 : 0;
+      chol_solve<T, NT>(G, e0, zt);
+#pragma unroll
+      for (int k = 0; k < NT; ++k) z[k] = (double)zt[k];
+      double last_dz = 0, zmax = 0;
+      for (int r = 0; r < NREF; ++r) {
+        double u[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) u[k] = 0;
+        for (int s = 0; s < ns; ++s) {
+          Geo<P2MAP> g;
+          load_geo(m, set[s], p2map, g);
+          for (int l = 0; l < q; ++l) {
+            double d[3], f[NT];
+            sample_d(m, g, p2map, l, xn, d);
+            basis<double, NT>(d, f);
+            double t = 0;
+#pragma unroll
+            for (int k = 0; k < NT; ++k) t = fma(f[k], z[k], t);
+#pragma unroll
+            for (int k = 0; k < NT; ++k) u[k] = fma(f[k], t, u[k]);
+          }
+        }
+        T res[NT], dz[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) res[k] = (T)((k == 0 ? 1.0 : 0.0) - u[k]);
+        chol_solve<T, NT>(G, res, dz);
+        last_dz = 0;
+        zmax = 0;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+          z[k] += (double)dz[k];
+          last_dz = fmax(last_dz, fabs((double)dz[k]));
+          zmax = fmax(zmax, fabs(z[k]));
+        }
+      }
+      if (NREF > 0 && !(last_dz <= 1e-13 * zmax) && poor != ACT_SOLVE) act = poor;
+    }
     wave_append(ACT_EXACT, act, a.L.count + 2 * NBUCKET, a.L.xq,
                 short_set ? (int32_t)p : (int32_t)(p | 0x80000000u));
     wave_append(ACT_FALLBACK, act, a.L.count + 2 * NBUCKET + 1, a.L.fbq,
                 expanded ? (int32_t)(id | 0x80000000u) : (int32_t)p);
+    for (int b = 0; b < NBUCKET; ++b)
+      wave_append(b, act == ACT_REFIT ? bucket : -1, a.L.count + b, a.L.items[b],
+                  expanded ? (int32_t)(id | 0x80000000u) : (int32_t)p);
     if (act == ACT_ILLCOND) atomicOr(a.L.status, ST_ILLCOND);
     if (act != ACT_SOLVE) continue;
-    // z = G^-1 e_0, refined against the FP64 residual
-    T e0[NT], zt[NT];
-#pragma unroll
-    for (int k = 0; k < NT; ++k) e0[k] = k == 0 ? 1 : 0;
-    chol_solve<T, NT>(G, e0, zt);
-    double z[NT];
-#pragma unroll
-    for (int k = 0; k < NT; ++k) z[k] = (double)zt[k];
-    for (int r = 0; r < NREF; ++r) {
-      double u[NT];
-#pragma unroll
-      for (int k = 0; k < NT; ++k) u[k] = 0;
-      for (int s = 0; s < ns; ++s) {
-        Geo<P2MAP> g;
-        load_geo(m, set[s], p2map, g);
-        for (int l = 0; l < q; ++l) {
-          double d[3], f[NT];
-          sample_d(m, g, p2map, l, xn, d);
-          basis<double, NT>(d, f);
-          double t = 0;
-#pragma unroll
-          for (int k = 0; k < NT; ++k) t = fma(f[k], z[k], t);
-#pragma unroll
-          for (int k = 0; k < NT; ++k) u[k] = fma(f[k], t, u[k]);
-        }
-      }
-      T res[NT], dz[NT];
-#pragma unroll
-      for (int k = 0; k < NT; ++k) res[k] = (T)((k == 0 ? 1.0 : 0.0) - u[k]);
-      chol_solve<T, NT>(G, res, dz);
-#pragma unroll
-      for (int k = 0; k < NT; ++k) z[k] += (double)dz[k];
-    }
     // weighted sum, centred on the first sample
     double v[NC], b0[NC];
     {
@@ -337,7 +364,11 @@ int launch_t(const FitArgs& a, int nterms, int src, float tau_rank, float tau_ac
 int launch_fast_fit(const FitArgs& a, int nterms, int form, int src, hipStream_t s)
 {
   if (a.q > NQ_MAX) return sprg_fail(SPRG_EUNSUPPORTED, "fast recovery: %d points per element", a.q);
-  if (src == 2) return launch_t<double, 2>(a, nterms, src, 0.0f, 1e-13f, s);
+  // FP64 passes refit sets whose smallest pivot ratio is below 1e-6 with the
+  // exact kernel.  Expanded sets (often barely full rank) always take the
+  // FP64 pass.
+  if (src == 2) return launch_t<double, 2>(a, nterms, src, 0.0f, 1e-6f, s);
+  if (src == 1) return launch_t<double, 0>(a, nterms, src, 0.0f, 1e-6f, s);
   // FP32: three FP64 refinement steps; each contracts the error by about
   // kappa(G) * 2^-24, and the size field needs the recovered field to ~1e-12
   if (form == SPRG_FP32R) return launch_t<float, 3>(a, nterms, src, 1e-4f, 1e-3f, s);

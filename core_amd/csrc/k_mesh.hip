@@ -114,7 +114,22 @@ __global__ void k_sort_rows(const int32_t* __restrict__ off, int64_t nrows, int3
 
 inline unsigned grid(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+__global__ void k_copy_i32(int32_t* __restrict__ dst, const int32_t* __restrict__ src, int64_t n)
+{
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src ? src[i] : 0;
+}
+
 }  // namespace
+
+// dst[0..n) = src[0..n), or 0 with src null.  A kernel rather than a
+// memset/memcpy node, so that sequences captured into a HIP graph replay
+// correctly.
+int launch_copy_i32(int32_t* dst, const int32_t* src, int64_t n, hipStream_t s)
+{
+  if (n > 0) k_copy_i32<<<grid(n, 256), 256, 0, s>>>(dst, src, n);
+  return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "copy launch failed");
+}
 
 // In-place inclusive scan of a[0..n); work holds ceil(n/2048) int32.
 int launch_scan_i32(int32_t* a, int64_t n, int32_t* work, hipStream_t s)

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "device.h"
@@ -65,6 +66,7 @@ struct sprg_mesh {
   hipEvent_t ev[NPHASE + 1] = {};
   bool ev_ok = false;
   double phase_ms[NPHASE] = {};
+  int64_t last_stats[5] = {};
   int phase_last = 0;  // highest phase recorded by the last call
   void mark(int i)
   {
@@ -292,6 +294,13 @@ extern "C" int sprg_mesh_phase_times(sprg_mesh* m, double* ms, int n)
   return NPHASE;
 }
 
+extern "C" int sprg_mesh_last_stats(sprg_mesh* m, int64_t* stats, int n)
+{
+  if (!m || (!stats && n > 0)) return sprg_fail(SPRG_EINVAL, "mesh_last_stats: null argument");
+  for (int i = 0; i < n && i < 5; ++i) stats[i] = m->last_stats[i];
+  return 5;
+}
+
 extern "C" int sprg_mesh_adjacency(sprg_mesh* m, int dim, const int32_t** off, const int32_t** idx)
 {
   if (!m || !off || !idx) return sprg_fail(SPRG_EINVAL, "mesh_adjacency: null argument");
@@ -381,70 +390,115 @@ int setup_lists(sprg_mesh* m, FitArgs& a)
 int read_status(sprg_mesh* m, const FitArgs& a, int32_t* st)
 {
   hipStream_t s = m->ctx->stream;
-  HIP_OK(hipMemcpyAsync(st, a.L.status, sizeof(int32_t), hipMemcpyDeviceToHost, s), "status");
+  // counts [0, 2 NBUCKET + 2) and the status word follow each other
+  int32_t c[2 * NBUCKET + 3];
+  HIP_OK(hipMemcpyAsync(c, a.L.count, sizeof c, hipMemcpyDeviceToHost, s), "status");
   HIP_OK(hipStreamSynchronize(s), "recover sync");
+  *st = c[2 * NBUCKET + 2];
+  int64_t expanded = 0;
+  for (int b = 0; b < NBUCKET; ++b) expanded += c[NBUCKET + b];
+  m->last_stats[0] = a.npatch;
+  m->last_stats[1] = c[2 * NBUCKET];
+  m->last_stats[2] = expanded;
+  m->last_stats[3] = c[2 * NBUCKET + 1];
+  m->last_stats[4] = *st;
   return 0;
 }
 
-// classify -> exact fit of the initial sets -> expansion -> exact fit of the
-// expanded sets.  Retries once with larger expanded-set storage.
-int run_recover(sprg_mesh* m, int q, int nc, const double* ip, double* out, int form)
+bool valid_form(int form) { return form == SPRG_FP64 || form == SPRG_FP32R || form == SPRG_FP64F; }
+
+// Enqueues one recovery on stream s: kernels only (no allocation, no host
+// synchronisation, counters cleared by a kernel rather than a memset node),
+// so the sequence can be captured into a HIP graph.  FP64 form: classify ->
+// exact fit of the initial sets -> expansion -> exact fit of the expanded
+// sets.  Fast forms: fit of every patch -> expansion -> fit of the expanded
+// sets -> FP64 fallback -> exact refits.  marks: record the phase events.
+int enqueue_recover(sprg_mesh* m, const FitArgs& a, int form, hipStream_t s, bool marks)
 {
-  if (form != SPRG_FP64 && form != SPRG_FP32R && form != SPRG_FP64F)
-    return sprg_fail(SPRG_EINVAL, "unknown numeric form %d", form);
   const bool fast = form != SPRG_FP64;
-  if (fast && q > 4)
-    return sprg_fail(SPRG_EUNSUPPORTED, "the fast forms take at most 4 points per element");
   const int nterms = m->order == 1 ? 4 : 10;
-  hipStream_t s = m->ctx->stream;
-  for (int attempt = 0; attempt < 4; ++attempt) {
-    FitArgs a;
-    a.m = m->view();
-    a.fast = fast ? 1 : 0;
-    a.q = q;
-    a.nc = nc;
-    a.npatch = m->npatch();
-    a.ip = ip;
-    a.out = out;
-    int rc = setup_lists(m, a);
-    if (rc) return rc;
-    HIP_OK(hipMemsetAsync(a.L.x_cursor, 0, sizeof(int64_t) * 2, s), "memset");
-    HIP_OK(hipMemsetAsync(a.L.count, 0, sizeof(int32_t) * (2 * NBUCKET + 4), s), "memset");
+  int rc;
+  // counters, cursors and status of the work lists
+  if ((rc = launch_copy_i32(a.L.count, nullptr, 2 * NBUCKET + 4, s))) return rc;
+  if ((rc = launch_copy_i32(reinterpret_cast<int32_t*>(a.L.x_cursor), nullptr, 4, s))) return rc;
+  if (marks) {
     m->phase_last = 0;
     m->mark(0);
-    if (fast) {
-      if ((rc = launch_fast_fit(a, nterms, form, 0, s))) return rc;
-    } else {
-      if ((rc = launch_classify(a, nterms, s))) return rc;
-      if ((rc = launch_exact_fit(a, nterms, false, s))) return rc;
-    }
-    m->mark(1);
-    if ((rc = launch_expand(a, nterms, s))) return rc;
-    m->mark(2);
-    if (fast) {
-      if ((rc = launch_fast_fit(a, nterms, form, 1, s))) return rc;
-      if ((rc = launch_fast_fit(a, nterms, form, 2, s))) return rc;
-    } else {
-      if ((rc = launch_exact_fit(a, nterms, true, s))) return rc;
-    }
-    m->mark(3);
+  }
+  if (fast) {
+    if ((rc = launch_fast_fit(a, nterms, form, 0, s))) return rc;
+  } else {
+    if ((rc = launch_classify(a, nterms, s))) return rc;
+    if ((rc = launch_exact_fit(a, nterms, 0, s))) return rc;
+  }
+  if (marks) m->mark(1);
+  if ((rc = launch_expand(a, nterms, s))) return rc;
+  if (marks) m->mark(2);
+  if (fast) {
+    if ((rc = launch_fast_fit(a, nterms, form, 1, s))) return rc;
+    if ((rc = launch_fast_fit(a, nterms, form, 2, s))) return rc;
+    // poorly conditioned sets: the reference's arithmetic
+    if ((rc = launch_exact_fit(a, nterms, 2, s))) return rc;
+  } else {
+    if ((rc = launch_exact_fit(a, nterms, 1, s))) return rc;
+  }
+  if (marks) m->mark(3);
+  return 0;
+}
+
+FitArgs fit_args(sprg_mesh* m, int q, int nc, const double* ip, double* out, int form)
+{
+  FitArgs a;
+  a.m = m->view();
+  a.fast = form != SPRG_FP64 ? 1 : 0;
+  a.q = q;
+  a.nc = nc;
+  a.npatch = m->npatch();
+  a.ip = ip;
+  a.out = out;
+  return a;
+}
+
+// Status word of a finished recovery -> 0 or an SPRG_E* code with message;
+// *retry is set when the expanded-set storage overflowed (after growing it).
+int status_code(sprg_mesh* m, int32_t st, bool* retry)
+{
+  *retry = false;
+  if (st & ST_XSET_OVERFLOW) {
+    int64_t cap = 2 * m->xset_cap;
+    int rc = m->xset.ensure(sizeof(int32_t) * cap);
+    if (rc) return rc;
+    m->xset_cap = cap;
+    *retry = true;
+    return 0;
+  }
+  if (st & ST_PATCH_FAIL) return sprg_fail(SPRG_EPATCH, "SPR: patch construction: all hope is lost.");
+  if (st & ST_TOO_LARGE)
+    return sprg_fail(SPRG_EUNSUPPORTED, "SPR: a patch needs more than %d sample rows", bucket_rows(NBUCKET - 1));
+  if (st & ST_INTERNAL) return sprg_fail(SPRG_EINTERNAL, "SPR: inconsistent rank decision");
+  if (st & ST_ILLCOND)
+    return sprg_fail(SPRG_EUNSUPPORTED,
+                     "SPR: a patch is too ill-conditioned for the fast forms; use SPRG_FP64");
+  return 0;
+}
+
+int run_recover(sprg_mesh* m, int q, int nc, const double* ip, double* out, int form)
+{
+  if (!valid_form(form)) return sprg_fail(SPRG_EINVAL, "unknown numeric form %d", form);
+  if (form != SPRG_FP64 && q > 4)
+    return sprg_fail(SPRG_EUNSUPPORTED, "the fast forms take at most 4 points per element");
+  hipStream_t s = m->ctx->stream;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    FitArgs a = fit_args(m, q, nc, ip, out, form);
+    int rc = setup_lists(m, a);
+    if (rc) return rc;
+    if ((rc = enqueue_recover(m, a, form, s, true))) return rc;
     int32_t st = 0;
     if ((rc = read_status(m, a, &st))) return rc;
     collect_phases(m);
-    if (st & ST_XSET_OVERFLOW) {
-      int64_t cap = 2 * m->xset_cap;
-      if ((rc = m->xset.ensure(sizeof(int32_t) * cap))) return rc;
-      m->xset_cap = cap;
-      continue;
-    }
-    if (st & ST_PATCH_FAIL) return sprg_fail(SPRG_EPATCH, "SPR: patch construction: all hope is lost.");
-    if (st & ST_TOO_LARGE)
-      return sprg_fail(SPRG_EUNSUPPORTED, "SPR: a patch needs more than %d sample rows", bucket_rows(NBUCKET - 1));
-    if (st & ST_INTERNAL) return sprg_fail(SPRG_EINTERNAL, "SPR: inconsistent rank decision");
-    if (st & ST_ILLCOND)
-      return sprg_fail(SPRG_EUNSUPPORTED,
-                       "SPR: a patch is too ill-conditioned for the fast forms; use SPRG_FP64");
-    return 0;
+    bool retry;
+    if ((rc = status_code(m, st, &retry))) return rc;
+    if (!retry) return 0;
   }
   return sprg_fail(SPRG_ENOMEM, "SPR: expanded patch storage kept overflowing");
 }
@@ -482,22 +536,22 @@ extern "C" int sprg_recover(sprg_mesh* m, int ip_order, int nc, const double* ip
 
 namespace {
 
-int run_estimate(sprg_mesh* m, int ip_order, int nc, const double* ip, int target,
-                 double ar, double tn, double alpha, double beta, double* size, int form,
-                 double* diag)
+// Buffers of one size-field call (recovered field, element sizes, block
+// partials, scalars): allocated before any capture.
+int estimate_buffers(sprg_mesh* m, int nc)
 {
-  auto t0 = std::chrono::steady_clock::now();
-  int q, rc = check_ip(m, ip_order, nc, &q);
-  if (rc) return rc;
-  if (nc != 1 && nc != 3 && nc != 9 && nc != 27)
-    return sprg_fail(SPRG_EUNSUPPORTED, "estimator: %d components", nc);
-  HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  int rc;
+  const int64_t nblocks = (m->nt + 255) / 256;
   if ((rc = m->rec.ensure(sizeof(double) * m->nnodes * nc))) return rc;
-  if ((rc = run_recover(m, q, nc, ip, m->rec.as<double>(), form))) return rc;
-  int64_t nblocks = (m->nt + 255) / 256;
   if ((rc = m->elem.ensure(sizeof(double) * 2 * m->nt))) return rc;
   if ((rc = m->partial.ensure(sizeof(double) * 2 * nblocks))) return rc;
   if ((rc = m->scalars.ensure(sizeof(double) * 4))) return rc;
+  return 0;
+}
+
+EstArgs est_args(sprg_mesh* m, int ip_order, int nc, const double* ip, int target, double ar,
+                 double tn, double alpha, double beta, double* size)
+{
   EstArgs e;
   e.m = m->view();
   e.ip_order = ip_order;
@@ -513,7 +567,23 @@ int run_estimate(sprg_mesh* m, int ip_order, int nc, const double* ip, int targe
   e.partial = m->partial.as<double>();
   e.scalars = m->scalars.as<double>();
   e.size = size;
-  e.nblocks = (int)nblocks;
+  e.nblocks = (int)((m->nt + 255) / 256);
+  return e;
+}
+
+int run_estimate(sprg_mesh* m, int ip_order, int nc, const double* ip, int target,
+                 double ar, double tn, double alpha, double beta, double* size, int form,
+                 double* diag)
+{
+  auto t0 = std::chrono::steady_clock::now();
+  int q, rc = check_ip(m, ip_order, nc, &q);
+  if (rc) return rc;
+  if (nc != 1 && nc != 3 && nc != 9 && nc != 27)
+    return sprg_fail(SPRG_EUNSUPPORTED, "estimator: %d components", nc);
+  HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  if ((rc = estimate_buffers(m, nc))) return rc;
+  if ((rc = run_recover(m, q, nc, ip, m->rec.as<double>(), form))) return rc;
+  EstArgs e = est_args(m, ip_order, nc, ip, target, ar, tn, alpha, beta, size);
   if ((rc = launch_estimate(e, m->ctx->stream, [m](int i) { m->mark(i); }))) return rc;
   double sc[4] = {0, 0, 0, 0};
   HIP_OK(hipMemcpyAsync(sc, e.scalars, sizeof(double) * 3, hipMemcpyDeviceToHost, m->ctx->stream),
@@ -645,4 +715,180 @@ extern "C" int sprg_patch_sets(sprg_mesh* m, int ip_order, int64_t* set_off, int
   k_set_fill<<<g, 256, 0, s>>>(a, len, xmap, set_off, set_idx, cap);
   HIP_OK(hipStreamSynchronize(s), "sync");
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batches: the size-field pipelines of several meshes recorded into one HIP
+// graph and replayed.
+
+struct sprg_batch {
+  sprg_ctx* ctx = nullptr;
+  std::vector<sprg_mesh*> meshes;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> key;  // what the recorded graph was built for
+  DBuf bstat;                 // [nmesh] status word of each mesh's recovery
+  void drop()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key.clear();
+  }
+};
+
+extern "C" int sprg_batch_create(sprg_ctx* ctx, int nmesh, sprg_mesh* const* meshes,
+                                 sprg_batch** out)
+{
+  if (!ctx || !out || nmesh <= 0 || !meshes) return sprg_fail(SPRG_EINVAL, "batch_create: bad argument");
+  *out = nullptr;
+  for (int i = 0; i < nmesh; ++i) {
+    if (!meshes[i]) return sprg_fail(SPRG_EINVAL, "batch_create: mesh %d is null", i);
+    if (meshes[i]->ctx != ctx)
+      return sprg_fail(SPRG_EINVAL, "batch_create: mesh %d belongs to another context", i);
+    for (int j = 0; j < i; ++j)
+      if (meshes[j] == meshes[i])
+        return sprg_fail(SPRG_EINVAL, "batch_create: mesh %d appears twice (meshes own their workspaces)", i);
+  }
+  HIP_OK(hipSetDevice(ctx->device), "hipSetDevice");
+  sprg_batch* b = new (std::nothrow) sprg_batch;
+  if (!b) return sprg_fail(SPRG_ENOMEM, "batch_create: out of host memory");
+  b->ctx = ctx;
+  b->meshes.assign(meshes, meshes + nmesh);
+  int rc = b->bstat.ensure(sizeof(int32_t) * nmesh);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" int sprg_batch_destroy(sprg_batch* b)
+{
+  if (!b) return 0;
+  hipSetDevice(b->ctx->device);
+  hipStreamSynchronize(b->ctx->stream);
+  b->drop();
+  b->bstat.release();
+  delete b;
+  return 0;
+}
+
+namespace {
+
+uint64_t bits(double x)
+{
+  uint64_t u;
+  memcpy(&u, &x, sizeof u);
+  return u;
+}
+
+// Records the pipelines of every mesh of the batch on the context stream.
+int record_batch(sprg_batch* b, int ip_order, int nc, const double* const* ip,
+                 const double* target, double ar, double alpha, double beta,
+                 double* const* size, int form)
+{
+  hipStream_t s = b->ctx->stream;
+  const int n = (int)b->meshes.size();
+  int q = rule_npoints(ip_order);
+  // every allocation happens before the capture
+  std::vector<FitArgs> fa(n);
+  for (int i = 0; i < n; ++i) {
+    sprg_mesh* m = b->meshes[i];
+    int rc = estimate_buffers(m, nc);
+    if (!rc) {
+      fa[i] = fit_args(m, q, nc, ip[i], m->rec.as<double>(), form);
+      rc = setup_lists(m, fa[i]);
+    }
+    if (rc) return rc;
+  }
+  HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+  int rc = 0;
+  for (int i = 0; i < n && !rc; ++i) {
+    sprg_mesh* m = b->meshes[i];
+    rc = enqueue_recover(m, fa[i], form, s, false);
+    if (!rc) {
+      const bool tgt = target[i] > 0;
+      EstArgs e = est_args(m, ip_order, nc, ip[i], tgt ? 1 : 0, ar, tgt ? target[i] : 0, alpha,
+                           beta, size[i]);
+      rc = launch_estimate(e, s, [](int) {});
+    }
+    if (!rc) rc = launch_copy_i32(b->bstat.as<int32_t>() + i, fa[i].L.status, 1, s);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess || !g) return sprg_fail(SPRG_EHIP, "batch: capture failed: %s", hipGetErrorString(e));
+  b->graph = g;
+  if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    b->drop();
+    return sprg_fail(SPRG_EHIP, "batch: graph instantiation failed");
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int sprg_batch_size(sprg_batch* b, int ip_order, int nc, const double* const* ip,
+                               const double* target, double adapt_ratio, double alpha, double beta,
+                               double* const* size, int form, double* ms)
+{
+  if (!b || !ip || !target || !size) return sprg_fail(SPRG_EINVAL, "batch_size: null argument");
+  if (!valid_form(form)) return sprg_fail(SPRG_EINVAL, "unknown numeric form %d", form);
+  if (nc != 1 && nc != 3 && nc != 9 && nc != 27)
+    return sprg_fail(SPRG_EUNSUPPORTED, "estimator: %d components", nc);
+  const int n = (int)b->meshes.size();
+  bool any_target = false;
+  for (int i = 0; i < n; ++i) {
+    int q, rc = check_ip(b->meshes[i], ip_order, nc, &q);
+    if (rc) return rc;
+    if (form != SPRG_FP64 && q > 4)
+      return sprg_fail(SPRG_EUNSUPPORTED, "the fast forms take at most 4 points per element");
+    if (!ip[i] || !size[i]) return sprg_fail(SPRG_EINVAL, "batch_size: null field of mesh %d", i);
+    any_target = any_target || target[i] > 0;
+  }
+  if (any_target && !(alpha < beta))
+    return sprg_fail(SPRG_EINVAL, "getTargetSPRSizeField: alpha must be < beta");
+  HIP_OK(hipSetDevice(b->ctx->device), "hipSetDevice");
+  std::vector<uint64_t> key = {(uint64_t)ip_order, (uint64_t)nc, (uint64_t)form, bits(adapt_ratio),
+                               bits(alpha), bits(beta)};
+  for (int i = 0; i < n; ++i) {
+    key.push_back((uint64_t)(uintptr_t)ip[i]);
+    key.push_back((uint64_t)(uintptr_t)size[i]);
+    key.push_back(bits(target[i]));
+  }
+  hipStream_t s = b->ctx->stream;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    if (!b->exec || key != b->key) {
+      b->drop();
+      int rc = record_batch(b, ip_order, nc, ip, target, adapt_ratio, alpha, beta, size, form);
+      if (rc) return rc;
+      b->key = key;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_OK(hipGraphLaunch(b->exec, s), "graph launch");
+    HIP_OK(hipStreamSynchronize(s), "batch sync");
+    auto t1 = std::chrono::steady_clock::now();
+    if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    std::vector<int32_t> st(n);
+    HIP_OK(hipMemcpy(st.data(), b->bstat.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost), "status");
+    bool again = false;
+    for (int i = 0; i < n; ++i) {
+      bool retry;
+      int rc = status_code(b->meshes[i], st[i], &retry);
+      if (rc) {
+        const std::string msg = sprg_last_error();
+        return sprg_fail(rc, "batch mesh %d: %s", i, msg.c_str());
+      }
+      again = again || retry;
+    }
+    if (!again) return 0;
+    b->drop();  // a mesh's expanded-set storage grew: record again
+  }
+  return sprg_fail(SPRG_ENOMEM, "SPR: expanded patch storage kept overflowing");
 }

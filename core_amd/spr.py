@@ -153,6 +153,13 @@ class Mesh:
         _lib.lib().sprg_mesh_phase_times(self.handle, ms.ctypes.data, 5)
         return ms
 
+    def last_stats(self):
+        """Patch routes of the last recovery: patches, queued for the exact
+        rank test / expansion, expanded sets, FP64 fallback refits, status."""
+        st = np.zeros(5, np.int64)
+        _lib.lib().sprg_mesh_last_stats(self.handle, st.ctypes.data, 5)
+        return dict(zip(["patches", "queued_exact", "expanded", "fallback", "status"], st.tolist()))
+
     def patch_sets(self, ip_order):
         """(off, idx) of the tet set every recovered entity is fitted on."""
         L = _lib.lib()
@@ -257,3 +264,58 @@ def getTargetSPRSizeField(f, t, alpha=0.25, beta=2.0, form="fp64", diag=None):
     if diag is not None:
         diag.update(global_error=d[0], size_factor=d[1], wall_ms=d[3])
     return out
+
+
+class Batch:
+    """Several meshes whose size-field pipelines run as one recorded HIP
+    graph (sprg_batch_*): the first call records, identical calls replay."""
+
+    def __init__(self, meshes):
+        meshes = list(meshes)
+        if not meshes:
+            raise SPRError(_lib.SPRG_EINVAL, "a batch needs at least one mesh")
+        self.meshes = meshes
+        self._arr = (_lib.ctypes.c_void_p * len(meshes))(*[m.handle.value for m in meshes])
+        h = _lib.ctypes.c_void_p()
+        _lib.check(_lib.lib().sprg_batch_create(meshes[0].ctx, len(meshes), self._arr,
+                                                _lib.ctypes.byref(h)))
+        self.handle = h
+        self.last_ms = None
+
+    def close(self):
+        if self.handle:
+            _lib.lib().sprg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def size_fields(self, fields, targets=None, adapt_ratio=0.1, alpha=0.25, beta=2.0,
+                    form="fp64", outputs=None):
+        """For mesh i: getTargetSPRSizeField(fields[i], targets[i], alpha,
+        beta) when targets[i] > 0, else getSPRSizeField(fields[i],
+        adapt_ratio).  Returns the "size" tensors; pass the previous
+        `outputs` back to replay the recorded graph."""
+        n = len(self.meshes)
+        if len(fields) != n:
+            raise SPRError(_lib.SPRG_EINVAL, "one field per mesh")
+        f0 = fields[0]
+        for f, m in zip(fields, self.meshes):
+            if f.kind != "ip" or f.mesh is not m or f.order != f0.order or f.ncomp != f0.ncomp:
+                raise SPRError(_lib.SPRG_EINVAL, "batch fields: IP fields of the batch's meshes, "
+                                                 "one order and component count")
+        if outputs is None:
+            outputs = [m._tensor((m.nv,)) for m in self.meshes]
+        tg = np.asarray([0.0] * n if targets is None else targets, dtype=np.float64)
+        ips = (_lib.ctypes.c_void_p * n)(*[f.data.data_ptr() for f in fields])
+        outs = (_lib.ctypes.c_void_p * n)(*[o.data_ptr() for o in outputs])
+        ms = np.zeros(1)
+        _sync_torch()
+        _lib.check(_lib.lib().sprg_batch_size(self.handle, f0.order, f0.ncomp, ips, tg.ctypes.data,
+                                              float(adapt_ratio), float(alpha), float(beta), outs,
+                                              _form(form), ms.ctypes.data))
+        self.last_ms = float(ms[0])
+        return outputs

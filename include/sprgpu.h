@@ -97,6 +97,11 @@ int sprg_mesh_counts(sprg_mesh* mesh, int64_t* counts);
  * Writes min(n, 5) values and returns 5.  No reference counterpart (the
  * reference times the whole call, spr/sprEstimateError.cc:380-386). */
 int sprg_mesh_phase_times(sprg_mesh* mesh, double* ms, int n);
+/* Patch routes of the last recovery on the mesh: [0] patches, [1] patches
+ * queued for the exact rank test / expansion, [2] expanded sets fitted,
+ * [3] patches refitted by the fast forms' FP64 fallback, [4] status bits.
+ * Writes min(n, 5) values and returns 5.  No reference counterpart. */
+int sprg_mesh_last_stats(sprg_mesh* mesh, int64_t* stats, int n);
 /* Device CSR of getAdjacent(entity, 3): dim 0 (vertices) or 1 (edges).
  * off has n+1 entries, idx has off[n]; both stay owned by the mesh. */
 int sprg_mesh_adjacency(sprg_mesh* mesh, int dim, const int32_t** off,
@@ -136,6 +141,24 @@ int sprg_spr_size(sprg_mesh* mesh, int ip_order, int nc, const double* ip,
 int sprg_target_spr_size(sprg_mesh* mesh, int ip_order, int nc,
                          const double* ip, double target, double alpha,
                          double beta, double* size, int form, double* diag);
+
+/* ---- many meshes per launch sequence ------------------------------------
+ * A batch holds meshes of one context (each at most once).  sprg_batch_size
+ * runs, for every mesh i, spr::getTargetSPRSizeField(ip[i], target[i],
+ * alpha, beta) when target[i] > 0, else spr::getSPRSizeField(ip[i],
+ * adapt_ratio), writing size[i] [nv_i] (device pointers).  The first call
+ * records all pipelines into one HIP graph; calls with the same fields and
+ * parameters replay it (others record again).  ms (optional): wall time of
+ * the replay.  Results equal the per-mesh calls bit for bit.  No reference
+ * counterpart: the reference estimates one mesh per call. */
+typedef struct sprg_batch sprg_batch;
+int sprg_batch_create(sprg_ctx* ctx, int nmesh, sprg_mesh* const* meshes,
+                      sprg_batch** out);
+int sprg_batch_destroy(sprg_batch* batch);
+int sprg_batch_size(sprg_batch* batch, int ip_order, int nc,
+                    const double* const* ip, const double* target,
+                    double adapt_ratio, double alpha, double beta,
+                    double* const* size, int form, double* ms);
 
 /* ---- synthetic box meshes (host; mds/apfBox.cc:178-336 numbering) ------ */
 int sprg_box_counts(int n, int64_t* nv, int64_t* nt, int64_t* ne);
