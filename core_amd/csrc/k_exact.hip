@@ -418,22 +418,33 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
           const int32_t t = sOld[w >> 2];
           const int32_t* tv = m.tv + 4 * (int64_t)t;
           const int32_t v = tv[w & 3];
-          for (int32_t k = m.v2t_off[v]; k < m.v2t_off[v + 1]; ++k) {
-            const int32_t u = m.v2t[k];
-            if (shared_verts(m.tv + 4 * (int64_t)u, tv) < sd + 1) continue;
-            // past 256 distinct tets the set is too large anyway; stopping
-            // here keeps the hash far from full so every probe ends
-            if (*(volatile int*)&sN >= 256) break;
-            uint32_t h = ((uint32_t)u * 2654435761u) & (HSIZE - 1);
-            for (;;) {
-              int32_t prev = atomicCAS(&sHash[h], -1, u);
-              if (prev == -1) {
-                int pos = atomicAdd(&sN, 1);
-                if (pos < 256) sCur[pos] = u;
-                break;
+          const int32_t k0 = m.v2t_off[v], k1 = m.v2t_off[v + 1];
+          // neighbour ids and their vertices eight at a time, so the
+          // dependent loads overlap
+          for (int32_t k = k0; k < k1; k += 8) {
+            int32_t u[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) u[j] = (k + j < k1) ? m.v2t[k + j] : -1;
+            int sh[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sh[j] = u[j] >= 0 ? shared_verts(m.tv + 4 * (int64_t)u[j], tv) : 0;
+            for (int j = 0; j < 8; ++j) {
+              if (u[j] < 0 || sh[j] < sd + 1) continue;
+              // past 256 distinct tets the set is too large anyway;
+              // stopping here keeps the hash far from full so every probe
+              // ends
+              if (*(volatile int*)&sN >= 256) break;
+              uint32_t h = ((uint32_t)u[j] * 2654435761u) & (HSIZE - 1);
+              for (;;) {
+                int32_t prev = atomicCAS(&sHash[h], -1, u[j]);
+                if (prev == -1) {
+                  int pos = atomicAdd(&sN, 1);
+                  if (pos < 256) sCur[pos] = u[j];
+                  break;
+                }
+                if (prev == u[j]) break;
+                h = (h + 1) & (HSIZE - 1);
               }
-              if (prev == u) break;
-              h = (h + 1) & (HSIZE - 1);
             }
           }
         }

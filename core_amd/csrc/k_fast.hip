@@ -58,6 +58,54 @@ __device__ __forceinline__ void load_geo(const MeshView& m, int32_t t, bool p2ma
     }
 }
 
+// Calls f(t, geometry of t) for the tets of a set in order.  Straight-sided
+// meshes run a three-stage load pipeline: while tet s is processed, the
+// coordinates of tet s+1, the vertex ids of tet s+2 and the id of tet s+3
+// are in flight, so each dependent load (set -> vertex ids -> coordinates)
+// has a whole tet's arithmetic to arrive.  Ids past the end repeat a valid
+// tet (loaded, never used).
+template <bool P2MAP, class F>
+__device__ __forceinline__ void for_each_tet(const MeshView& m, const int32_t* set, int ns, F&& f)
+{
+  if constexpr (P2MAP) {
+    for (int s = 0; s < ns; ++s) {
+      Geo<true> g;
+      load_geo(m, set[s], true, g);
+      f(set[s], g);
+    }
+  } else {
+    auto ids = [&](int32_t t) { return *reinterpret_cast<const int4*>(m.tv + 4 * (int64_t)t); };
+    auto coords = [&](const int4& v, Geo<false>& g) {
+      const int32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const double* p = m.xyz + 3 * (int64_t)vv[a];
+        g.x[a][0] = p[0];
+        g.x[a][1] = p[1];
+        g.x[a][2] = p[2];
+      }
+    };
+    int32_t t0 = set[0];
+    int32_t t1 = ns > 1 ? set[1] : t0;
+    int32_t t2 = ns > 2 ? set[2] : t1;
+    int4 v1 = ids(t1);
+    Geo<false> g0;
+    coords(ids(t0), g0);
+    for (int s = 0; s < ns; ++s) {
+      Geo<false> g1;
+      coords(v1, g1);
+      const int4 v2 = ids(t2);
+      const int32_t t3 = s + 3 < ns ? set[s + 3] : t2;
+      f(t0, g0);
+      t0 = t1;
+      t1 = t2;
+      t2 = t3;
+      v1 = v2;
+      g0 = g1;
+    }
+  }
+}
+
 // Sample point l of the mesh-order rule, minus the node position.
 template <bool P2MAP>
 __device__ __forceinline__ void sample_d(const MeshView& m, const Geo<P2MAP>& g, bool p2map,
@@ -185,9 +233,7 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
       xn[1] = m.xyz[3 * p + 1];
       xn[2] = m.xyz[3 * p + 2];
       // pass 1: Gram matrix, packed lower triangle
-      for (int s = 0; s < ns; ++s) {
-        Geo<P2MAP> g;
-        load_geo(m, set[s], p2map, g);
+      for_each_tet<P2MAP>(m, set, ns, [&](int32_t, const Geo<P2MAP>& g) {
         for (int l = 0; l < q; ++l) {
           double d[3];
           sample_d(m, g, p2map, l, xn, d);
@@ -198,7 +244,7 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
 #pragma unroll
             for (int j = 0; j <= i; ++j) G[tri(i, j)] = fma(f[i], f[j], G[tri(i, j)]);
         }
-      }
+      });
       // Cholesky in place, with the rank certificate and the conditioning
 #pragma unroll
       for (int k = 0; k < NT; ++k) {
@@ -258,9 +304,7 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
         double u[NT];
 #pragma unroll
         for (int k = 0; k < NT; ++k) u[k] = 0;
-        for (int s = 0; s < ns; ++s) {
-          Geo<P2MAP> g;
-          load_geo(m, set[s], p2map, g);
+        for_each_tet<P2MAP>(m, set, ns, [&](int32_t, const Geo<P2MAP>& g) {
           for (int l = 0; l < q; ++l) {
             double d[3], f[NT];
             sample_d(m, g, p2map, l, xn, d);
@@ -271,7 +315,7 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
 #pragma unroll
             for (int k = 0; k < NT; ++k) u[k] = fma(f[k], t, u[k]);
           }
-        }
+        });
         T res[NT], dz[NT];
 #pragma unroll
         for (int k = 0; k < NT; ++k) res[k] = (T)((k == 0 ? 1.0 : 0.0) - u[k]);
@@ -303,22 +347,22 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
 #pragma unroll
       for (int c = 0; c < NC; ++c) { b0[c] = bp[c]; v[c] = 0; }
     }
-    for (int s = 0; s < ns; ++s) {
-      Geo<P2MAP> g;
-      const int32_t t = set[s];
-      load_geo(m, t, p2map, g);
+    for_each_tet<P2MAP>(m, set, ns, [&](int32_t t, const Geo<P2MAP>& g) {
       for (int l = 0; l < q; ++l) {
+        const double* bp = a.ip + ((int64_t)t * q + l) * NC;
+        double b[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) b[c] = bp[c];  // issued before the weight
         double d[3], f[NT];
         sample_d(m, g, p2map, l, xn, d);
         basis<double, NT>(d, f);
         double w = 0;
 #pragma unroll
         for (int k = 0; k < NT; ++k) w = fma(f[k], z[k], w);
-        const double* bp = a.ip + ((int64_t)t * q + l) * NC;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) v[c] = fma(w, bp[c] - b0[c], v[c]);
+        for (int c = 0; c < NC; ++c) v[c] = fma(w, b[c] - b0[c], v[c]);
       }
-    }
+    });
     double* out = a.out + p * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) out[c] = b0[c] + v[c];
