@@ -146,6 +146,17 @@ def cpu_baseline(host, ip, budget_s, workers):
     return out
 
 
+def traffic_of(form):
+    """HBM bytes per launch of the fit pass from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE measurement (scaled by the copy probe), for the
+    form it was measured on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "k_fast_strain120_traffic.json")
+    if form != "fp64f" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["traffic_bytes"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,7 +260,7 @@ def main():
                       "element": phases[3], "factor_average": phases[4]},
         "routes": m.last_stats(),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(args.form),
                      "measured_copy_gbs": probe["gbs"],
                      "kernel": ("k_fast regular pass" if args.form != "fp64" else "exact fit")
                                + " (all %d patches), %d algorithmic bytes per launch, %.3f ms"
