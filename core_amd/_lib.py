@@ -51,6 +51,7 @@ SIGNATURES = {
     "sprg_spr_size": (_c_int, [_vp, _c_int, _c_int, _vp, _c_dbl, _vp, _c_int, _vp]),
     "sprg_target_spr_size": (_c_int, [_vp, _c_int, _c_int, _vp, _c_dbl, _c_dbl, _c_dbl, _vp,
                                       _c_int, _vp]),
+    "sprg_average_to_vertex": (_c_int, [_vp, _vp, _vp]),
     "sprg_batch_create": (_c_int, [_vp, _c_int, _vp, _pp]),
     "sprg_batch_destroy": (_c_int, [_vp]),
     "sprg_batch_size": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_dbl, _c_dbl, _c_dbl, _vp, _c_int,

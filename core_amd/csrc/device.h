@@ -129,6 +129,7 @@ struct EstArgs {
 // mark(i) records phase event i on the stream (3: before the element pass,
 // 4: after it, 5: after the vertex average).
 int launch_estimate(EstArgs& a, hipStream_t s, const std::function<void(int)>& mark);
+int launch_vertex_mean(const MeshView& m, const double* elem, double* out, hipStream_t s);
 
 }  // namespace sprg
 

@@ -181,7 +181,27 @@ __global__ void k_average(EstArgs a)
   a.size[v] = s;
 }
 
+// spr::averageToVertex for every vertex (spr/sprEstimateError.cc:255-265):
+// the mean of an element scalar over the vertex's tets, summed in ascending
+// tet order.
+__global__ void k_vertex_mean(MeshView m, const double* __restrict__ elem, double* __restrict__ out)
+{
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= m.nv) return;
+  const int32_t b = m.v2t_off[v], e = m.v2t_off[v + 1];
+  double s = 0;
+  for (int32_t k = b; k < e; ++k) s += elem[m.v2t[k]];
+  s /= (double)(e - b);
+  out[v] = s;
+}
+
 }  // namespace
+
+int launch_vertex_mean(const MeshView& m, const double* elem, double* out, hipStream_t s)
+{
+  if (m.nv > 0) k_vertex_mean<<<(unsigned)((m.nv + 255) / 256), 256, 0, s>>>(m, elem, out);
+  return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "vertex mean launch failed");
+}
 
 int launch_estimate(EstArgs& a, hipStream_t s, const std::function<void(int)>& mark)
 {

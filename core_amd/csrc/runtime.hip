@@ -615,6 +615,16 @@ extern "C" int sprg_spr_size(sprg_mesh* m, int ip_order, int nc, const double* i
   return run_estimate(m, ip_order, nc, ip, 0, adapt_ratio, 0, 0, 0, size, form, diag);
 }
 
+extern "C" int sprg_average_to_vertex(sprg_mesh* m, const double* elem, double* vert)
+{
+  if (!m || !elem || !vert) return sprg_fail(SPRG_EINVAL, "average_to_vertex: null argument");
+  HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  int rc = launch_vertex_mean(m->view(), elem, vert, m->ctx->stream);
+  if (rc) return rc;
+  HIP_OK(hipStreamSynchronize(m->ctx->stream), "average sync");
+  return 0;
+}
+
 extern "C" int sprg_target_spr_size(sprg_mesh* m, int ip_order, int nc, const double* ip,
                                     double target, double alpha, double beta, double* size,
                                     int form, double* diag)

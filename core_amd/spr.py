@@ -266,6 +266,22 @@ def getTargetSPRSizeField(f, t, alpha=0.25, beta=2.0, form="fp64", diag=None):
     return out
 
 
+def averageToVertex(element_values, mesh, name="size"):
+    """spr::averageToVertex over every vertex: a vertex field (Lagrange-1
+    scalar) holding the mean of an element scalar over the vertex's tets."""
+    if element_values.shape[0] != mesh.nt:
+        raise SPRError(_lib.SPRG_EINVAL, "one value per tet expected")
+    ev = element_values.reshape(mesh.nt).to(device="cuda:%d" % mesh.device, dtype=torch.float64).contiguous()
+    out = mesh.createLagrangeField(name, 1, 1)
+    _sync_torch()
+    try:
+        _lib.check(_lib.lib().sprg_average_to_vertex(mesh.handle, _dptr(ev), _dptr(out.data)))
+    except SPRError:
+        mesh.destroyField(out)
+        raise
+    return out
+
+
 class Batch:
     """Several meshes whose size-field pipelines run as one recorded HIP
     graph (sprg_batch_*): the first call records, identical calls replay."""

@@ -142,6 +142,11 @@ int sprg_target_spr_size(sprg_mesh* mesh, int ip_order, int nc,
                          const double* ip, double target, double alpha,
                          double beta, double* size, int form, double* diag);
 
+/* ---- spr::averageToVertex (spr/sprEstimateError.cc:255-265) -----------
+ * elem [nt] (an element scalar field) -> vert [nv]: the mean over each
+ * vertex's tets (getAdjacent(vertex, 3), summed in ascending tet order). */
+int sprg_average_to_vertex(sprg_mesh* mesh, const double* elem, double* vert);
+
 /* ---- many meshes per launch sequence ------------------------------------
  * A batch holds meshes of one context (each at most once).  sprg_batch_size
  * runs, for every mesh i, spr::getTargetSPRSizeField(ip[i], target[i],

@@ -7,6 +7,7 @@ held to 1e-13 relative (the north-star bar for the FP64 form is 1e-12).
 """
 import numpy as np
 import pytest
+import torch
 
 from core_amd import _lib, mesh, spr, workloads
 from oracle import oracle as O
@@ -136,6 +137,24 @@ def test_target_size(case):
         m.destroyField(f)
     ref, _ = O.target_size(om, 2, ipv, 2 * h.nt, 0.25, 2.0)
     assert np.max(np.abs(got / ref - 1)) <= 1e-13
+
+
+def test_average_to_vertex(case):
+    # spr::averageToVertex (spr/sprEstimateError.cc:255-265): mean of an
+    # element scalar over the vertex's tets, in the adjacency order
+    h, m, om = case
+    rng = np.random.default_rng(7)
+    ev = rng.uniform(0.5, 2.0, h.nt)
+    got = spr.averageToVertex(torch.from_numpy(ev), m, name="avg").numpy()[:, 0]
+    m.destroyField(m.findField("avg"))
+    off, idx = O.vertex_tets(om)
+    ref = np.empty(h.nv)
+    for v in range(h.nv):
+        s = 0.0
+        for k in range(off[v], off[v + 1]):
+            s += ev[idx[k]]
+        ref[v] = s / (off[v + 1] - off[v])
+    assert np.array_equal(got, ref)
 
 
 def test_patch_failure_is_reported(gpu):
