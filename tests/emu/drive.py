@@ -1,0 +1,134 @@
+"""Runs the CPU emulation of the library (tests/emu/build.py) on tiny P1
+meshes and checks it against the oracle: adjacency, patch sets, gradients
+and recovered fields bit for bit, size fields to 1e-13, the fast forms to
+1e-6 on the size field, and a two-mesh batch (recorded graph, replayed)
+equal to the per-mesh calls.  (Each block's lanes are OS threads meeting at
+condition-variable barriers, so P2 meshes take many minutes here; the GPU
+tests cover them.)  Started by tests/test_emu_cpu.py in a child
+process with AddressSanitizer preloaded; prints one line per check and exits
+non-zero on the first failure."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from core_amd import _lib  # noqa: E402
+
+_lib.LIB_PATH = os.path.join(HERE, "_build", "libsprgpu_emu.so")
+from core_amd import mesh, workloads  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+L = _lib.lib()
+vp = ctypes.c_void_p
+
+
+def P(a):
+    return vp(a.ctypes.data)
+
+
+def check(name, ok):
+    print(("PASS " if ok else "FAIL ") + name, flush=True)
+    if not ok:
+        sys.exit(1)
+
+
+def device_mesh(ctx, h):
+    m = vp()
+    _lib.check(L.sprg_mesh_create(ctx, h.nv, h.nt, h.ne if h.te is not None else 0, P(h.tv),
+                                  None if h.te is None else P(h.te), P(h.xyz), h.order,
+                                  ctypes.byref(m)))
+    return m
+
+
+def exact_sequence(ctx, name, h):
+    om = O.Mesh(h.tv, h.xyz, h.order, te=h.te, ev=h.ev)
+    m = device_mesh(ctx, h)
+    nodes = h.nv + (h.ne if h.order == 2 else 0)
+    for dim in ([0, 1] if h.order == 2 else [0]):
+        off, idx = vp(), vp()
+        _lib.check(L.sprg_mesh_adjacency(m, dim, ctypes.byref(off), ctypes.byref(idx)))
+        n = h.nv if dim == 0 else h.ne
+        o = np.frombuffer((ctypes.c_int32 * (n + 1)).from_address(off.value), np.int32).copy()
+        i = np.frombuffer((ctypes.c_int32 * int(o[-1])).from_address(idx.value), np.int32).copy()
+        ro, ri = O.vertex_tets(om) if dim == 0 else O.edge_tets(om)
+        check("%s adjacency %d" % (name, dim), np.array_equal(o, ro) and np.array_equal(i, ri))
+    for ipo, q in ((1, 1), (2, 4)):
+        off = np.zeros(nodes + 1, np.int64)
+        total = np.zeros(1, np.int64)
+        _lib.check(L.sprg_patch_sets(m, ipo, P(off), None, 0, P(total)))
+        idx = np.zeros(max(int(total[0]), 1), np.int32)
+        _lib.check(L.sprg_patch_sets(m, ipo, P(off), P(idx), idx.size, P(total)))
+        ro, ri = O.patch_sets(om, q)
+        check("%s patch sets q=%d" % (name, q),
+              np.array_equal(off, ro.astype(np.int64)) and np.array_equal(idx[:total[0]], ri))
+    Pn = workloads.node_points(h)
+    u = np.ascontiguousarray(np.stack([workloads.sin_field(Pn), workloads.exp_field(Pn),
+                                       Pn[:, 0] * Pn[:, 1]], 1))
+    g = np.full((h.nt, 4, 9), np.nan)
+    _lib.check(L.sprg_grad_ip(m, h.order, 3, P(u), 2, P(g)))
+    check("%s gradient" % name, np.array_equal(g, O.grad_ip(om, h.order, u, 2)))
+    ip = np.ascontiguousarray(workloads.strain_field(h.ip_points(2)))
+    out = np.full((nodes, 9), np.nan)
+    _lib.check(L.sprg_recover(m, 2, 9, P(ip), P(out), 0))
+    check("%s recovered field" % name, np.array_equal(out, O.recover(om, ip)))
+    ref, _ = O.spr_size(om, 2, ip, 0.05)
+    for form, tol in ((0, 1e-13), (1, 1e-6), (2, 1e-6)):
+        size = np.full(h.nv, np.nan)
+        diag = np.zeros(4)
+        _lib.check(L.sprg_spr_size(m, 2, 9, P(ip), 0.05, P(size), form, P(diag)))
+        check("%s size field form %d" % (name, form), np.max(np.abs(size / ref - 1)) <= tol)
+    L.sprg_mesh_destroy(m)
+
+
+def batch_check(ctx):
+    # two P1 meshes, one with the target estimator and one with the
+    # adapt-ratio estimator, in one recorded graph
+    hs, ms, eps, eager, tg = [], [], [], [], []
+    for i, n in enumerate([2, 3]):
+        h = mesh.box(n, order=1, jitter=0.2, seed=workloads.C4 + i)
+        m = device_mesh(ctx, h)
+        u = np.ascontiguousarray(workloads.sin_field(h.xyz).reshape(-1, 1))
+        g = np.full((h.nt, 4, 3), np.nan)
+        _lib.check(L.sprg_grad_ip(m, 1, 1, P(u), 2, P(g)))
+        s = np.full(h.nv, np.nan)
+        diag = np.zeros(4)
+        if i == 0:
+            t = 2.0 * h.nt
+            _lib.check(L.sprg_target_spr_size(m, 2, 3, P(g), t, 0.25, 2.0, P(s), 2, P(diag)))
+        else:
+            t = 0.0
+            _lib.check(L.sprg_spr_size(m, 2, 3, P(g), 0.1, P(s), 2, P(diag)))
+        hs.append(h), ms.append(m), eps.append(g), eager.append(s), tg.append(t)
+    arr = (vp * 2)(*[m.value for m in ms])
+    b = vp()
+    _lib.check(L.sprg_batch_create(ctx, 2, arr, ctypes.byref(b)))
+    outs = [np.full(h.nv, -1.0) for h in hs]
+    ips = (vp * 2)(*[e.ctypes.data for e in eps])
+    ops = (vp * 2)(*[o.ctypes.data for o in outs])
+    tga = np.array(tg)
+    t = np.zeros(1)
+    for k in range(2):
+        for o in outs:
+            o[:] = -1.0
+        _lib.check(L.sprg_batch_size(b, 2, 3, ips, P(tga), 0.1, 0.25, 2.0, ops, 2, P(t)))
+        check("batch run %d equals the per-mesh calls" % k,
+              all(np.array_equal(o, e) for o, e in zip(outs, eager)))
+    L.sprg_batch_destroy(b)
+    for m in ms:
+        L.sprg_mesh_destroy(m)
+
+
+def main():
+    ctx = vp()
+    _lib.check(L.sprg_ctx_create(0, ctypes.byref(ctx)))
+    exact_sequence(ctx, "p1 n=2 jittered", mesh.box(2, order=1, jitter=0.25, seed=11))
+    batch_check(ctx)
+    print("ALL PASS", flush=True)
+
+
+if __name__ == "__main__":
+    main()
