@@ -169,6 +169,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--exact-steps", type=int, default=2)
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose the patch kernels' launch shapes by timing before the timed steps")
+    ap.add_argument("--tune-cache", default=None,
+                    help="JSON cache of tuned launch shapes (read, and written after tuning)")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run the 1 GiB device copy probe and exit")
     args = ap.parse_args()
@@ -204,6 +208,11 @@ def main():
     eps = m.createIPField("eps", 9, 2, ip)
     torch.cuda.synchronize()
     log("uploaded (%.1fs)" % (time.time() - t0))
+    shape = None
+    if args.autotune:
+        t1 = time.time()
+        shape = m.autotune(eps, args.form, cache=args.tune_cache)
+        log("autotuned launch shapes %s (%.1fs)" % (shape, time.time() - t1))
 
     def step(form):
         s = spr.getSPRSizeField(eps, 0.05, form=form)
@@ -255,7 +264,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "strain_120: getSPRSizeField(0.05) on n=120 P2 jittered cube, "
                                "10,368,000 tets, 9-comp strain IP field, IP order 2",
-                   "form": args.form, "tets": T, "vertices": V, "edges": E, "patches": patches},
+                   "form": args.form, "tets": T, "vertices": V, "edges": E, "patches": patches,
+                   "launch_shape": shape or "default"},
         "phases_ms": {"fit": phases[0], "expand": phases[1], "expanded_fit": phases[2],
                       "element": phases[3], "factor_average": phases[4]},
         "routes": m.last_stats(),

@@ -45,6 +45,8 @@ SIGNATURES = {
     "sprg_mesh_adjacency": (_c_int, [_vp, _c_int, _pp, _pp]),
     "sprg_mesh_phase_times": (_c_int, [_vp, _vp, _c_int]),
     "sprg_mesh_last_stats": (_c_int, [_vp, _vp, _c_int]),
+    "sprg_mesh_autotune": (_c_int, [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int]),
+    "sprg_mesh_set_shape": (_c_int, [_vp, _vp, _c_int]),
     "sprg_grad_ip": (_c_int, [_vp, _c_int, _c_int, _vp, _c_int, _vp]),
     "sprg_patch_sets": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp]),
     "sprg_recover": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int]),

@@ -55,6 +55,9 @@ struct Lists {
   int32_t* xitems[NBUCKET];
   int32_t* xq;               // patches queued for the exact rank test /
                              // expansion; bit 31: test the initial set first
+  int32_t* xq_exact;         // queued patches whose expansion needs the exact
+                             // rank test (same encoding as xq); its count
+                             // follows the status word
   int32_t* fbq;              // fast forms: FP64 fallback queue; bit 31: an
                              // expanded item rather than a patch id
   // expanded sets
@@ -62,13 +65,24 @@ struct Lists {
   int64_t* x_off;
   int32_t* x_len;
   int32_t* x_set;
-  int64_t* x_cursor;         // [0] set entries used, [1] items used
+  int64_t* x_cursor;         // [0] set entries reserved, [1] item ids (one
+                             // per queued patch; x_len 0: no set stored)
   int64_t x_cap;             // capacity of x_set
   int64_t cap;               // capacity of each list (>= number of patches)
   int32_t* status;
 };
 
+// Launch shapes of the patch kernels (chosen per mesh by sprg_mesh_autotune;
+// every shape computes every patch with the same arithmetic).
+struct LaunchShape {
+  int fast_threads = 256;   // k_fast block size (64, 128 or 256)
+  int fast_blocks = 8192;   // k_fast persistent grid cap
+  int exact_blocks = 8192;  // k_exact_fit grid per bucket (a quarter for 192 rows)
+  int expand_blocks = 4096; // k_expand grid
+};
+
 struct FitArgs {
+  LaunchShape shape;
   MeshView m;
   int fast;        // fast forms: one regular list, one expanded list
   int q;           // points per element

@@ -38,15 +38,26 @@ __device__ __forceinline__ int64_t node_of(const MeshView& m, int32_t t, int n)
 
 // getSamplePoints (spr/sprRecoverField.cc:193-211): the rule of the MESH
 // order, point l (the N1 rule ignores l), mapped by the coordinate field
-// (apf/apfElement.cc:106-114 summation order).
+// (apf/apfElement.cc:106-114 summation order).  The mesh order is fixed at
+// compile time (a P1 mesh fits 4 terms, a P2 mesh 10), so the rule point and
+// shape values stay in registers.
+template <int ORDER>
 __device__ void sample_point(const MeshView& m, int32_t t, int l, double* x)
 {
-  double xi[3], N[10];
-  rule_point(m.order, l, xi);
-  tet_values(m.order, xi, N);
-  int nn = tet_nodes(m.order);
+  constexpr int NN = ORDER == 1 ? 4 : 10;
+  double xi[3], N[NN];
+  if (ORDER == 1) {
+    xi[0] = 0.25; xi[1] = 0.25; xi[2] = 0.25;
+  } else {  // rule_point(2, l)
+    const double a = 0.138196601125011, b = 0.585410196624969;
+    xi[0] = l == 1 ? b : a;
+    xi[1] = l == 2 ? b : a;
+    xi[2] = l == 3 ? b : a;
+  }
+  tet_values(ORDER, xi, N);
   x[0] = 0; x[1] = 0; x[2] = 0;
-  for (int ni = 0; ni < nn; ++ni) {
+#pragma unroll
+  for (int ni = 0; ni < NN; ++ni) {
     const double* p = m.xyz + 3 * node_of(m, t, ni);
     x[0] += p[0] * N[ni];
     x[1] += p[1] * N[ni];
@@ -61,7 +72,7 @@ __device__ void build_rows(const MeshView& m, int q, const int32_t* S, int nrows
 {
   for (int i = gl; i < nrows; i += LANES) {
     double x[3], t[10];
-    sample_point(m, S[i / q], i % q, x);
+    sample_point<NT == 4 ? 1 : 2>(m, S[i / q], i % q, x);
     poly_terms(NT == 4 ? 1 : 2, x, t);
     for (int j = 0; j < NT; ++j) R[i * NT + j] = t[j];
   }
@@ -297,7 +308,7 @@ __global__ __launch_bounds__(64) void k_exact_fit(FitArgs a, int bucket)
 
 constexpr int XMAXR = 192;   // rows of a patch the expansion can test
 constexpr int XSET = 192;    // tets in a set
-constexpr int HSIZE = 2048;  // LDS hash of candidate tets
+constexpr int HSIZE = 1024;  // LDS hash of candidate tets (at most 319 entries)
 
 __device__ __forceinline__ int shared_verts(const int32_t* a, const int32_t* b)
 {
@@ -307,68 +318,96 @@ __device__ __forceinline__ int shared_verts(const int32_t* a, const int32_t* b)
   return s;
 }
 
+// Monomial term k of poly_terms as v[a] * v[b] over v = (1, x, y, z): the
+// same products poly_terms forms (a unit factor is exact).
+__device__ __forceinline__ double term_of(int a, int b, const double* d)
+{
+  const double va = a == 0 ? 1.0 : a == 1 ? d[0] : a == 2 ? d[1] : d[2];
+  const double vb = b == 0 ? 1.0 : b == 1 ? d[0] : b == 2 ? d[1] : d[2];
+  return va * vb;
+}
+
+__device__ __forceinline__ void term_factors(int k, int* a, int* b)
+{
+  // 1, x, y, z, xy, yz, zx, xx, yy, zz
+  *a = k < 4 ? k : k < 7 ? k - 3 : k - 6;
+  *b = k < 4 ? 0 : k == 4 ? 2 : k == 5 ? 3 : k == 6 ? 1 : k - 6;
+}
+
 // Rank screen of a candidate set, one wave: the FP64 Gram matrix of the
-// monomials centred at the patch node, summed over the rows lane by lane and
-// across the wave (butterfly, so every lane holds the same sums), then its
-// Cholesky pivots.  Centring is a unit-triangular change of the graded
-// monomial basis, so in exact arithmetic pivot k equals the reference's
-// Householder column norm R_kk (mth/mthQR.cc:31).  Returns true only when
-// every pivot clears the reference's 1e-10 by 5% and none lost more than 11
-// of the diagonal's 16 digits to cancellation; then the reference's test
-// passes too.  A false return decides nothing: the exact test runs.
+// monomials centred at the patch node and its Cholesky pivots.  The rows'
+// centred sample points go to LDS (sD, 3 doubles per row); lane l < NT(NT+1)/2
+// owns Gram entry l of the packed lower triangle, sums it over the rows and
+// keeps it through a right-looking factorisation whose columns travel by
+// lane shuffles (the fused multiply-adds of each entry run in the same order
+// as a left-looking Cholesky's).  Centring is a unit-triangular change of the
+// graded monomial basis, so in exact arithmetic pivot k equals the
+// reference's Householder column norm R_kk (mth/mthQR.cc:31).  Returns true
+// only when every pivot clears the reference's 1e-10 by 5% and keeps at least
+// 1e-6 of its diagonal; then the reference's test passes too.  (The rounding
+// of a pivot grows with the cancellation before it: on an exactly singular
+// 10x10 P2 set whose previous pivot kept 1.6e-8 of its diagonal, the last
+// pivot came out at +-5e-11 of its diagonal, so a bound near 1e-11 does not
+// separate rank-deficient sets from full ones.)  A false return decides
+// nothing: the exact test runs.
+// Every lane of the block (one wave) must call it.
 template <int NT>
 __device__ bool certify_full_rank(const MeshView& m, int q, const int32_t* S, int nrows,
-                                  int64_t p, int lane)
+                                  int64_t p, int lane, double* sD)
 {
   constexpr int NG = NT * (NT + 1) / 2;
-  double G[NG];
-#pragma unroll
-  for (int k = 0; k < NG; ++k) G[k] = 0;
   const double xn[3] = {m.xyz[3 * p], m.xyz[3 * p + 1], m.xyz[3 * p + 2]};
   for (int i = lane; i < nrows; i += 64) {
-    double x[3], d[3], f[10];
-    sample_point(m, S[i / q], i % q, x);
-    for (int j = 0; j < 3; ++j) d[j] = x[j] - xn[j];
-    poly_terms(NT == 4 ? 1 : 2, d, f);
-    int k = 0;
-#pragma unroll
-    for (int r = 0; r < NT; ++r)
-#pragma unroll
-      for (int c = 0; c <= r; ++c, ++k) G[k] = fma(f[r], f[c], G[k]);
+    double x[3];
+    sample_point<NT == 4 ? 1 : 2>(m, S[i / q], i % q, x);
+    for (int j = 0; j < 3; ++j) sD[3 * i + j] = x[j] - xn[j];
   }
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1)
-#pragma unroll
-    for (int k = 0; k < NG; ++k) G[k] += __shfl_xor(G[k], off);
+  __syncthreads();
+  const bool own = lane < NG;
+  int r = 0;
+  while (own && (r + 1) * (r + 2) / 2 <= lane) ++r;
+  const int c = own ? lane - r * (r + 1) / 2 : 0;
+  int ra, rb, ca, cb;
+  term_factors(r, &ra, &rb);
+  term_factors(c, &ca, &cb);
+  double g = 0;
+  if (own)
+    for (int i = 0; i < nrows; ++i) {
+      const double d[3] = {sD[3 * i], sD[3 * i + 1], sD[3 * i + 2]};
+      g = fma(term_of(ra, rb, d), term_of(ca, cb, d), g);
+    }
+  __syncthreads();  // sD is free again
+  const double diag_own = g;
   bool rank_ok = true;
-#pragma unroll
   for (int k = 0; k < NT; ++k) {
     const int kk = k * (k + 1) / 2 + k;
-    const double diag = G[kk];
-    double dk = diag;
-#pragma unroll
-    for (int j = 0; j < k; ++j) dk = fma(-G[k * (k + 1) / 2 + j], G[k * (k + 1) / 2 + j], dk);
+    const double dk = __shfl(g, kk);
+    const double diag = __shfl(diag_own, kk);
     const double lkk = dk > 0 ? sqrt(dk) : 0.0;
-    if (!(diag > 0 && dk >= 1e-11 * diag) || !(lkk >= 1.05e-10)) rank_ok = false;
-    G[kk] = lkk > 0 ? lkk : 1.0;
-#pragma unroll
-    for (int i = k + 1; i < NT; ++i) {
-      double sacc = G[i * (i + 1) / 2 + k];
-#pragma unroll
-      for (int j = 0; j < k; ++j) sacc = fma(-G[i * (i + 1) / 2 + j], G[k * (k + 1) / 2 + j], sacc);
-      G[i * (i + 1) / 2 + k] = sacc / G[kk];
-    }
+    if (!(diag > 0 && dk >= 1e-6 * diag) || !(lkk >= 1.05e-10)) rank_ok = false;
+    const double piv = lkk > 0 ? lkk : 1.0;
+    if (own && r > k && c == k) g = g / piv;  // L[r][k]
+    const double lr = __shfl(g, own && r > k ? r * (r + 1) / 2 + k : 0);
+    const double lc = __shfl(g, own && c > k ? c * (c + 1) / 2 + k : 0);
+    if (own && c > k) g = fma(-lr, lc, g);
   }
   return rank_ok;
 }
 
-template <int NT>
+// EXACT = false: the screen alone, with no QR workspace in LDS (several
+// times the occupancy); a patch whose growing set the screen cannot certify,
+// or that must test its initial set first, is deferred whole to xq_exact.
+// EXACT = true: walks xq_exact with the screen backed by the reference's
+// test.  Each patch's expansion depends only on the mesh, so deferring it
+// changes nothing but where it runs.
+template <int NT, bool EXACT>
 __global__ __launch_bounds__(64) void k_expand(FitArgs a)
 {
-  __shared__ double sR[XMAXR * NT];
-  __shared__ double sV[NT * XMAXR];
+  __shared__ double sR[EXACT ? XMAXR * NT : 1];
+  __shared__ double sV[EXACT ? NT * XMAXR : 1];
   __shared__ double sDot[NT];
   __shared__ double sNorm[2];
+  __shared__ double sD[3 * XMAXR];
   __shared__ int32_t sCur[256];
   __shared__ int32_t sOld[XSET];
   __shared__ int32_t sHash[HSIZE];
@@ -376,9 +415,23 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
   const int lane = threadIdx.x;
   const MeshView& m = a.m;
   const int q = a.q;
-  const int nq = a.L.count[2 * NBUCKET];
-  for (int it = blockIdx.x; it < nq; it += gridDim.x) {
+  int32_t* const nexact = a.L.status + 1;
+  const int32_t* const queue = EXACT ? a.L.xq_exact : a.L.xq;
+  const int nq = EXACT ? *nexact : a.L.count[2 * NBUCKET];
+  // expanded sets are stored in chunks of x_set that each block reserves
+  // with one atomic (a per-patch atomic on one address serialises the
+  // launch); a chunk is at most 1024 entries and a quarter of x_set over the
+  // grid
+  __shared__ int64_t sBase, sLeft;
+  const int64_t share = a.L.x_cap / (4 * (int64_t)gridDim.x);
+  const int64_t chunk_size = share > 1024 ? 1024 : share < XSET ? XSET : share;
+  if (lane == 0) { sBase = 0; sLeft = 0; }
+  for (int k = blockIdx.x; k < nq; k += gridDim.x) {
+    // the item id of a queued patch is its position in xq; the light pass
+    // marks it empty first (x_len 0) and whichever pass finds the set fills it
+    const int32_t it = EXACT ? queue[k] : k;
     const int32_t entry = a.L.xq[it];
+    if (!EXACT && lane == 0) a.L.x_len[it] = 0;
     const int64_t p = entry & 0x7fffffff;
     const bool test_first = entry < 0;
     const int32_t* set0;
@@ -391,10 +444,15 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
       __syncthreads();
       continue;
     }
+    if (!EXACT && test_first && ns0 * q >= NT) {
+      if (lane == 0) a.L.xq_exact[atomicAdd(nexact, 1)] = it;
+      __syncthreads();
+      continue;
+    }
     for (int i = lane; i < ns0; i += 64) sCur[i] = set0[i];
     __syncthreads();
-    bool done = false, failed = false;
-    if (test_first && ns0 * q >= NT) {
+    bool done = false, failed = false, deferred = false;
+    if (EXACT && test_first && ns0 * q >= NT) {
       // a fast form could not certify the rank: run the reference's test
       const int mrows = ns0 * q;
       if (mrows > XMAXR) {
@@ -406,7 +464,7 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
         done = r_recurrence<NT, 64, XMAXR>(sR, sV, sDot, sNorm, mrows, lane, true);
       }
     }
-    for (int round = 0; round < 64 && !done && !failed; ++round) {
+    for (int round = 0; round < 64 && !done && !failed && !deferred; ++round) {
       const int nold = sN;
       for (int i = lane; i < nold; i += 64) sOld[i] = sCur[i];
       __syncthreads();
@@ -475,7 +533,12 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
         bool ok = false;
         if (mrows >= NT && mrows <= XMAXR) {
           // a certified full rank needs no exact test; anything else does
-          ok = certify_full_rank<NT>(m, q, sCur, mrows, p, lane);
+          ok = certify_full_rank<NT>(m, q, sCur, mrows, p, lane, sD);
+          if (!ok && !EXACT) {
+            if (lane == 0) a.L.xq_exact[atomicAdd(nexact, 1)] = it;
+            deferred = true;
+            break;
+          }
           if (!ok) {
             build_rows<NT, 64>(m, q, sCur, mrows, sR, lane);
             __syncthreads();
@@ -489,34 +552,63 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
         }
         if (ok) done = true;
       }
-      if (!done && !failed && sN <= nold) failed = true;
+      if (!done && !failed && !deferred && sN <= nold) failed = true;
     }
     if (failed && lane == 0)
       atomicOr(a.L.status, sOverflow ? ST_TOO_LARGE : ST_PATCH_FAIL);
     if (done) {
       __shared__ int64_t sOff;
-      __shared__ int32_t sItem;
       const int ns = sN;
       if (lane == 0) {
-        sOff = atomicAdd((unsigned long long*)a.L.x_cursor, (unsigned long long)ns);
-        sItem = (int32_t)atomicAdd((unsigned long long*)(a.L.x_cursor + 1), 1ull);
+        if (sLeft < ns) {
+          const int64_t grab = ns > chunk_size ? (int64_t)ns : chunk_size;
+          sBase = (int64_t)atomicAdd((unsigned long long*)a.L.x_cursor, (unsigned long long)grab);
+          sLeft = grab;
+        }
+        sOff = sBase;
+        sBase += ns;
+        sLeft -= ns;
       }
       __syncthreads();
-      if (sOff + ns > a.L.x_cap || sItem >= a.L.cap) {
+      if (sOff + ns > a.L.x_cap) {
         if (lane == 0) atomicOr(a.L.status, ST_XSET_OVERFLOW);
       } else {
         for (int i = lane; i < ns; i += 64) a.L.x_set[sOff + i] = sCur[i];
         if (lane == 0) {
-          a.L.x_patch[sItem] = (int32_t)p;
-          a.L.x_off[sItem] = sOff;
-          a.L.x_len[sItem] = ns;
-          int b = a.fast ? 0 : bucket_of(ns * q);
-          int k = atomicAdd(a.L.count + NBUCKET + b, 1);
-          a.L.xitems[b][k] = sItem;
+          a.L.x_patch[it] = (int32_t)p;
+          a.L.x_off[it] = sOff;
+          a.L.x_len[it] = ns;
         }
       }
     }
     __syncthreads();
+  }
+}
+
+// The expanded sets found (x_len > 0) into the lists the fits walk: by row
+// bucket in the FP64 form, one list in the fast forms.  Wave-aggregated
+// appends; also publishes the item count (one id per queued patch).
+__global__ __launch_bounds__(256) void k_bucket_items(FitArgs a)
+{
+  const int nq = a.L.count[2 * NBUCKET];
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.L.x_cursor[1] = nq;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // the trip count is the same for every lane of a block (convergent appends)
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nq; base += stride) {
+    const int64_t it = base + threadIdx.x;
+    int target = -1;
+    if (it < nq) {
+      const int len = a.L.x_len[it];
+      if (len > 0) {
+        target = a.fast ? 0 : bucket_of(len * a.q);
+        if (target >= NBUCKET) {
+          atomicOr(a.L.status, ST_TOO_LARGE);
+          target = -1;
+        }
+      }
+    }
+    for (int b = 0; b < NBUCKET; ++b)
+      wave_append(b, target, a.L.count + NBUCKET + b, a.L.xitems[b], (int32_t)it);
   }
 }
 
@@ -527,13 +619,14 @@ int fit_all(const FitArgs& a, hipStream_t s)
 {
   // Persistent grids: each block loops over its share of the bucket's
   // device-side item count.
-  const unsigned blocks = 8192;
+  const unsigned blocks = (unsigned)a.shape.exact_blocks;
+  const unsigned quarter = blocks / 4 > 0 ? blocks / 4 : 1;  // the 192-row bucket is rare
   k_exact_fit<NT, 16, 16, EXP><<<blocks, 64, 0, s>>>(a, 0);
   k_exact_fit<NT, 32, 32, EXP><<<blocks, 64, 0, s>>>(a, 1);
   k_exact_fit<NT, 64, 64, EXP><<<blocks, 64, 0, s>>>(a, 2);
   k_exact_fit<NT, 96, 64, EXP><<<blocks, 64, 0, s>>>(a, 3);
   k_exact_fit<NT, 128, 64, EXP><<<blocks, 64, 0, s>>>(a, 4);
-  k_exact_fit<NT, 192, 64, EXP><<<blocks / 4, 64, 0, s>>>(a, 5);
+  k_exact_fit<NT, 192, 64, EXP><<<quarter, 64, 0, s>>>(a, 5);
   return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "exact fit launch failed");
 }
 
@@ -557,11 +650,17 @@ int launch_exact_fit(const FitArgs& a, int nterms, int mode, hipStream_t s)
 
 int launch_expand(const FitArgs& a, int nterms, hipStream_t s)
 {
-  const unsigned blocks = 4096;
-  if (nterms == 4)
-    k_expand<4><<<blocks, 64, 0, s>>>(a);
-  else
-    k_expand<10><<<blocks, 64, 0, s>>>(a);
+  const unsigned blocks = (unsigned)a.shape.expand_blocks;
+  // the deferred patches are few: a smaller persistent grid
+  const unsigned exact_blocks = blocks < 1024 ? blocks : 1024;
+  if (nterms == 4) {
+    k_expand<4, false><<<blocks, 64, 0, s>>>(a);
+    k_expand<4, true><<<exact_blocks, 64, 0, s>>>(a);
+  } else {
+    k_expand<10, false><<<blocks, 64, 0, s>>>(a);
+    k_expand<10, true><<<exact_blocks, 64, 0, s>>>(a);
+  }
+  k_bucket_items<<<256, 256, 0, s>>>(a);
   return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "expand launch failed");
 }
 

@@ -372,9 +372,10 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
 template <class T, int NT, int NREF, bool P2>
 int launch_nc2(const FitArgs& a, int src, float tau_rank, float tau_acc, hipStream_t s)
 {
-  const unsigned threads = 256;
+  const unsigned threads = (unsigned)a.shape.fast_threads;
   const int64_t work = src == SRC_REGULAR ? a.npatch : a.npatch / 8 + 1;
-  const unsigned blocks = (unsigned)std::min<int64_t>(8192, (work + threads - 1) / threads);
+  const unsigned blocks =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>(a.shape.fast_blocks, (work + threads - 1) / threads));
   switch (a.nc) {
     case 1: k_fast<T, NT, 1, NREF, P2><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
     case 3: k_fast<T, NT, 3, NREF, P2><<<blocks, threads, 0, s>>>(a, src, tau_rank, tau_acc); break;
@@ -416,7 +417,10 @@ int launch_fast_fit(const FitArgs& a, int nterms, int form, int src, hipStream_t
   // FP32: three FP64 refinement steps; each contracts the error by about
   // kappa(G) * 2^-24, and the size field needs the recovered field to ~1e-12
   if (form == SPRG_FP32R) return launch_t<float, 3>(a, nterms, src, 1e-4f, 1e-3f, s);
-  return launch_t<double, 0>(a, nterms, src, 1e-11f, 1e-6f, s);
+  // a pivot that keeps less than 1e-6 of its diagonal is not a rank
+  // certificate in FP64 (its rounding can exceed 1e-11 of the diagonal after
+  // earlier cancellation): such sets take the reference's rank test
+  return launch_t<double, 0>(a, nterms, src, 1e-6f, 1e-6f, s);
 }
 
 }  // namespace sprg

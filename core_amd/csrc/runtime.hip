@@ -67,6 +67,7 @@ struct sprg_mesh {
   bool ev_ok = false;
   double phase_ms[NPHASE] = {};
   int64_t last_stats[5] = {};
+  LaunchShape shape;  // launch shapes of the patch kernels (sprg_mesh_autotune)
   int phase_last = 0;  // highest phase recorded by the last call
   void mark(int i)
   {
@@ -350,9 +351,9 @@ int setup_lists(sprg_mesh* m, FitArgs& a)
 {
   const int64_t np = m->npatch();
   if (m->list_cap < np) {
-    // items and xitems per bucket, xq, x_patch, x_len: int32; x_off: int64;
-    // counts, cursors, status
-    size_t n32 = (size_t)np * (2 * NBUCKET + 4) + 2 * NBUCKET + 8;
+    // items and xitems per bucket, xq, xq_exact, fbq, x_patch, x_len:
+    // int32; x_off: int64; counts, cursors, status
+    size_t n32 = (size_t)np * (2 * NBUCKET + 5) + 2 * NBUCKET + 8;
     size_t bytes = n32 * 4 + (size_t)np * 8 + 64;
     int rc = m->lists.ensure(bytes);
     if (rc) return rc;
@@ -378,6 +379,7 @@ int setup_lists(sprg_mesh* m, FitArgs& a)
   for (int b = 0; b < NBUCKET; ++b) { L.items[b] = p; p += m->list_cap; }
   for (int b = 0; b < NBUCKET; ++b) { L.xitems[b] = p; p += m->list_cap; }
   L.xq = p; p += m->list_cap;
+  L.xq_exact = p; p += m->list_cap;
   L.fbq = p; p += m->list_cap;
   L.x_patch = p; p += m->list_cap;
   L.x_len = p; p += m->list_cap;
@@ -418,7 +420,7 @@ int enqueue_recover(sprg_mesh* m, const FitArgs& a, int form, hipStream_t s, boo
   const bool fast = form != SPRG_FP64;
   const int nterms = m->order == 1 ? 4 : 10;
   int rc;
-  // counters, cursors and status of the work lists
+  // counters, status and the exact-expansion count of the work lists
   if ((rc = launch_copy_i32(a.L.count, nullptr, 2 * NBUCKET + 4, s))) return rc;
   if ((rc = launch_copy_i32(reinterpret_cast<int32_t*>(a.L.x_cursor), nullptr, 4, s))) return rc;
   if (marks) {
@@ -449,6 +451,7 @@ int enqueue_recover(sprg_mesh* m, const FitArgs& a, int form, hipStream_t s, boo
 FitArgs fit_args(sprg_mesh* m, int q, int nc, const double* ip, double* out, int form)
 {
   FitArgs a;
+  a.shape = m->shape;
   a.m = m->view();
   a.fast = form != SPRG_FP64 ? 1 : 0;
   a.q = q;
@@ -615,6 +618,86 @@ extern "C" int sprg_spr_size(sprg_mesh* m, int ip_order, int nc, const double* i
   return run_estimate(m, ip_order, nc, ip, 0, adapt_ratio, 0, 0, 0, size, form, diag);
 }
 
+// Sets the launch shapes of the patch kernels (for instance from a cache of
+// earlier sprg_mesh_autotune results).
+extern "C" int sprg_mesh_set_shape(sprg_mesh* m, const int64_t* shape, int n)
+{
+  if (!m || !shape || n < 4) return sprg_fail(SPRG_EINVAL, "set_shape: needs 4 values");
+  if (shape[0] != 64 && shape[0] != 128 && shape[0] != 256)
+    return sprg_fail(SPRG_EINVAL, "set_shape: fast-fit block size must be 64, 128 or 256");
+  for (int i = 1; i < 4; ++i)
+    if (shape[i] < 1 || shape[i] > (1 << 20))
+      return sprg_fail(SPRG_EINVAL, "set_shape: grid sizes must lie in [1, 2^20]");
+  m->shape.fast_threads = (int)shape[0];
+  m->shape.fast_blocks = (int)shape[1];
+  m->shape.exact_blocks = (int)shape[2];
+  m->shape.expand_blocks = (int)shape[3];
+  return SPRG_OK;
+}
+
+// Times recoveries of the field with candidate launch shapes, one parameter
+// at a time (coordinate descent, each candidate the best of two runs), and
+// keeps the fastest shape for the mesh's later calls.
+extern "C" int sprg_mesh_autotune(sprg_mesh* m, int ip_order, int nc, const double* ip, int form,
+                                  int64_t* shape, int n)
+{
+  if (!m || !ip) return sprg_fail(SPRG_EINVAL, "autotune: null argument");
+  int q, rc = check_ip(m, ip_order, nc, &q);
+  if (rc) return rc;
+  HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  if ((rc = m->rec.ensure(sizeof(double) * m->nnodes * nc))) return rc;
+  auto time_once = [&](double* ms) -> int {
+    auto t0 = std::chrono::steady_clock::now();
+    int r = run_recover(m, q, nc, ip, m->rec.as<double>(), form);
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return r;
+  };
+  auto measure = [&](double* best) -> int {
+    double a = 0, b = 0;
+    int r = time_once(&a);
+    if (!r) r = time_once(&b);
+    *best = a < b ? a : b;
+    return r;
+  };
+  double best_ms;
+  if ((rc = measure(&best_ms))) return rc;
+  const bool fast = form != SPRG_FP64;
+  struct Param {
+    int LaunchShape::*field;
+    std::vector<int> values;
+  };
+  std::vector<Param> params;
+  if (fast) {
+    params.push_back({&LaunchShape::fast_threads, {64, 128, 256}});
+    params.push_back({&LaunchShape::fast_blocks, {2048, 4096, 8192, 16384, 32768}});
+  } else {
+    params.push_back({&LaunchShape::exact_blocks, {2048, 4096, 8192, 16384}});
+  }
+  params.push_back({&LaunchShape::expand_blocks, {1024, 2048, 4096, 8192}});
+  for (const Param& prm : params) {
+    const int keep = m->shape.*prm.field;
+    int chosen = keep;
+    for (int v : prm.values) {
+      if (v == keep) continue;
+      m->shape.*prm.field = v;
+      double ms;
+      if ((rc = measure(&ms))) {
+        m->shape.*prm.field = keep;
+        return rc;
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        chosen = v;
+      }
+    }
+    m->shape.*prm.field = chosen;
+  }
+  const int64_t out[4] = {m->shape.fast_threads, m->shape.fast_blocks, m->shape.exact_blocks,
+                          m->shape.expand_blocks};
+  for (int i = 0; shape && i < n && i < 4; ++i) shape[i] = out[i];
+  return SPRG_OK;
+}
+
 extern "C" int sprg_average_to_vertex(sprg_mesh* m, const double* elem, double* vert)
 {
   if (!m || !elem || !vert) return sprg_fail(SPRG_EINVAL, "average_to_vertex: null argument");
@@ -659,7 +742,7 @@ __global__ void k_set_len(FitArgs a, int32_t* len, int32_t* xmap)
 __global__ void k_xmap(FitArgs a, int32_t* xmap, int64_t nitems)
 {
   int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= nitems) return;
+  if (k >= nitems || a.L.x_len[k] <= 0) return;  // no set found for this queued patch
   xmap[a.L.x_patch[k]] = (int32_t)k;
 }
 

@@ -17,6 +17,9 @@ these raise SPRError with the library's status code; a duplicate field name
 Device memory comes from torch; the library runs on its own HIP stream and
 every call returns after its work is complete.
 """
+import json
+import os
+
 import numpy as np
 import torch
 
@@ -32,6 +35,13 @@ _CTX = {}
 def _sync_torch():
     if torch.cuda.is_available():
         torch.cuda.current_stream().synchronize()
+
+
+def _read_json(path):
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        return json.load(fh)
 
 
 def context(device=0):
@@ -159,6 +169,40 @@ class Mesh:
         st = np.zeros(5, np.int64)
         _lib.lib().sprg_mesh_last_stats(self.handle, st.ctypes.data, 5)
         return dict(zip(["patches", "queued_exact", "expanded", "fallback", "status"], st.tolist()))
+
+    SHAPE_KEYS = ("fast_threads", "fast_blocks", "exact_blocks", "expand_blocks")
+
+    def autotune(self, ip_field, form="fp64f", cache=None):
+        """Chooses this mesh's patch-kernel launch shapes by timing
+        recoveries of `ip_field` (sprg_mesh_autotune) and returns them.
+        With `cache` (a JSON file path), shapes are looked up and stored
+        under the key "<gfx arch>/p<mesh order>/c<components>/q<points>/<form>"
+        and a cached entry is applied without timing."""
+        key = None
+        if cache is not None:
+            arch = torch.cuda.get_device_properties(self.device).gcnArchName.split(":")[0]
+            key = "%s/p%d/c%d/q%d/%s" % (arch, self.order, ip_field.ncomp,
+                                         ip_field.data.shape[1], form)
+            table = _read_json(cache)
+            if key in table:
+                self.set_shape(table[key])
+                return dict(table[key])
+        sh = np.zeros(4, np.int64)
+        _sync_torch()
+        _lib.check(_lib.lib().sprg_mesh_autotune(self.handle, ip_field.order, ip_field.ncomp,
+                                                 _dptr(ip_field.data), _form(form), sh.ctypes.data, 4))
+        shape = dict(zip(self.SHAPE_KEYS, sh.tolist()))
+        if key is not None:
+            table = _read_json(cache)
+            table[key] = shape
+            with open(cache, "w") as fh:
+                json.dump(table, fh, indent=1, sort_keys=True)
+        return shape
+
+    def set_shape(self, shape):
+        """Applies launch shapes as autotune() returns them."""
+        sh = np.array([int(shape[k]) for k in self.SHAPE_KEYS], np.int64)
+        _lib.check(_lib.lib().sprg_mesh_set_shape(self.handle, sh.ctypes.data, 4))
 
     def patch_sets(self, ip_order):
         """(off, idx) of the tet set every recovered entity is fitted on."""

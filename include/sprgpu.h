@@ -102,6 +102,19 @@ int sprg_mesh_phase_times(sprg_mesh* mesh, double* ms, int n);
  * [3] patches refitted by the fast forms' FP64 fallback, [4] status bits.
  * Writes min(n, 5) values and returns 5.  No reference counterpart. */
 int sprg_mesh_last_stats(sprg_mesh* mesh, int64_t* stats, int n);
+/* Chooses the launch shapes of the patch kernels for this mesh by timing
+ * recoveries of `ip` in `form` (coordinate descent over block sizes and
+ * persistent grid sizes) and keeps the fastest for the mesh's later calls.
+ * Every shape computes every patch with the same arithmetic, so results do
+ * not change.  Writes min(n, 4) values (fast-fit threads per block, fast-fit
+ * grid cap, exact-fit grid, expansion grid) to `shape`, which may be NULL.
+ * No reference counterpart. */
+int sprg_mesh_autotune(sprg_mesh* mesh, int ip_order, int nc, const double* ip,
+                       int form, int64_t* shape, int n);
+/* Sets the four launch-shape values sprg_mesh_autotune reports (block size
+ * 64/128/256; grids in [1, 2^20]), e.g. from a cache of tuned shapes.  No
+ * reference counterpart. */
+int sprg_mesh_set_shape(sprg_mesh* mesh, const int64_t* shape, int n);
 /* Device CSR of getAdjacent(entity, 3): dim 0 (vertices) or 1 (edges).
  * off has n+1 entries, idx has off[n]; both stay owned by the mesh. */
 int sprg_mesh_adjacency(sprg_mesh* mesh, int dim, const int32_t** off,

@@ -76,11 +76,22 @@ def exact_sequence(ctx, name, h):
     _lib.check(L.sprg_recover(m, 2, 9, P(ip), P(out), 0))
     check("%s recovered field" % name, np.array_equal(out, O.recover(om, ip)))
     ref, _ = O.spr_size(om, 2, ip, 0.05)
+    sizes = {}
     for form, tol in ((0, 1e-13), (1, 1e-6), (2, 1e-6)):
         size = np.full(h.nv, np.nan)
         diag = np.zeros(4)
         _lib.check(L.sprg_spr_size(m, 2, 9, P(ip), 0.05, P(size), form, P(diag)))
         check("%s size field form %d" % (name, form), np.max(np.abs(size / ref - 1)) <= tol)
+        sizes[form] = size
+    # launch shapes chosen by timing change no result (the fast form's tuner
+    # covers the fast-fit, exact-refit and expansion grids)
+    for form in (1,):
+        sh = np.zeros(4, np.int64)
+        _lib.check(L.sprg_mesh_autotune(m, 2, 9, P(ip), form, P(sh), 4))
+        size = np.full(h.nv, np.nan)
+        diag = np.zeros(4)
+        _lib.check(L.sprg_spr_size(m, 2, 9, P(ip), 0.05, P(size), form, P(diag)))
+        check("%s autotuned form %d %s" % (name, form, sh.tolist()), np.array_equal(size, sizes[form]))
     L.sprg_mesh_destroy(m)
 
 

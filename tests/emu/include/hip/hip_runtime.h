@@ -39,6 +39,15 @@ int64_t shfl64(int64_t v, int src);
 inline void __syncthreads() { emu::block_sync(); }
 inline uint64_t __ballot(int p) { return emu::ballot(p != 0); }
 inline int __shfl(int v, int src) { return (int)emu::shfl64(v, src); }
+inline double __shfl(double v, int src)
+{
+  int64_t b;
+  memcpy(&b, &v, 8);
+  b = emu::shfl64(b, src);
+  double r;
+  memcpy(&r, &b, 8);
+  return r;
+}
 inline double __shfl_xor(double v, int mask)
 {
   int64_t b;

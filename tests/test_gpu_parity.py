@@ -21,6 +21,9 @@ CASES = [
     ("p2", 3, 2, 0.0, 0, False),
     ("p2_jitter", 4, 2, 0.25, workloads.C5, False),
     ("p2_graded", 4, 2, 0.3, workloads.C3, True),
+    # one point per tet makes square 10x10 sets, some exactly singular: the
+    # rank screen must hand them to the reference's test
+    ("p2_n2", 2, 2, 0.25, 11, False),
 ]
 
 
