@@ -410,6 +410,7 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
   __shared__ double sD[3 * XMAXR];
   __shared__ int32_t sCur[256];
   __shared__ int32_t sOld[XSET];
+  __shared__ int32_t sTmp[XSET];
   __shared__ int32_t sHash[HSIZE];
   __shared__ int sN, sOverflow;
   const int lane = threadIdx.x;
@@ -514,21 +515,17 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
           failed = true;
           break;
         }
-        // bitonic sort of sCur[0..256) with padding
-        for (int i = ns + lane; i < 256; i += 64) sCur[i] = 0x7fffffff;
+        // sort sCur[0..ns) ascending by rank: the ids are distinct, so the
+        // position of each is the number of smaller ids (ns <= XSET here)
+        for (int i = lane; i < ns; i += 64) {
+          const int32_t v = sCur[i];
+          int r = 0;
+          for (int j = 0; j < ns; ++j) r += sCur[j] < v ? 1 : 0;
+          sTmp[r] = v;
+        }
         __syncthreads();
-        for (int size = 2; size <= 256; size <<= 1)
-          for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = lane; i < 256; i += 64) {
-              int j = i ^ stride;
-              if (j > i) {
-                bool up = (i & size) == 0;
-                int32_t x = sCur[i], y = sCur[j];
-                if ((x > y) == up) { sCur[i] = y; sCur[j] = x; }
-              }
-            }
-            __syncthreads();
-          }
+        for (int i = lane; i < ns; i += 64) sCur[i] = sTmp[i];
+        __syncthreads();
         const int mrows = ns * q;
         bool ok = false;
         if (mrows >= NT && mrows <= XMAXR) {

@@ -66,7 +66,7 @@ struct sprg_mesh {
   hipEvent_t ev[NPHASE + 1] = {};
   bool ev_ok = false;
   double phase_ms[NPHASE] = {};
-  int64_t last_stats[5] = {};
+  int64_t last_stats[6] = {};
   LaunchShape shape;  // launch shapes of the patch kernels (sprg_mesh_autotune)
   int phase_last = 0;  // highest phase recorded by the last call
   void mark(int i)
@@ -298,8 +298,8 @@ extern "C" int sprg_mesh_phase_times(sprg_mesh* m, double* ms, int n)
 extern "C" int sprg_mesh_last_stats(sprg_mesh* m, int64_t* stats, int n)
 {
   if (!m || (!stats && n > 0)) return sprg_fail(SPRG_EINVAL, "mesh_last_stats: null argument");
-  for (int i = 0; i < n && i < 5; ++i) stats[i] = m->last_stats[i];
-  return 5;
+  for (int i = 0; i < n && i < 6; ++i) stats[i] = m->last_stats[i];
+  return 6;
 }
 
 extern "C" int sprg_mesh_adjacency(sprg_mesh* m, int dim, const int32_t** off, const int32_t** idx)
@@ -392,8 +392,9 @@ int setup_lists(sprg_mesh* m, FitArgs& a)
 int read_status(sprg_mesh* m, const FitArgs& a, int32_t* st)
 {
   hipStream_t s = m->ctx->stream;
-  // counts [0, 2 NBUCKET + 2) and the status word follow each other
-  int32_t c[2 * NBUCKET + 3];
+  // counts [0, 2 NBUCKET + 2), the status word and the exact-expansion
+  // count follow each other
+  int32_t c[2 * NBUCKET + 4];
   HIP_OK(hipMemcpyAsync(c, a.L.count, sizeof c, hipMemcpyDeviceToHost, s), "status");
   HIP_OK(hipStreamSynchronize(s), "recover sync");
   *st = c[2 * NBUCKET + 2];
@@ -404,6 +405,7 @@ int read_status(sprg_mesh* m, const FitArgs& a, int32_t* st)
   m->last_stats[2] = expanded;
   m->last_stats[3] = c[2 * NBUCKET + 1];
   m->last_stats[4] = *st;
+  m->last_stats[5] = c[2 * NBUCKET + 3];
   return 0;
 }
 

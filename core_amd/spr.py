@@ -165,10 +165,12 @@ class Mesh:
 
     def last_stats(self):
         """Patch routes of the last recovery: patches, queued for the exact
-        rank test / expansion, expanded sets, FP64 fallback refits, status."""
-        st = np.zeros(5, np.int64)
-        _lib.lib().sprg_mesh_last_stats(self.handle, st.ctypes.data, 5)
-        return dict(zip(["patches", "queued_exact", "expanded", "fallback", "status"], st.tolist()))
+        rank test / expansion, expanded sets, FP64 fallback refits, status,
+        queued patches whose expansion needed the reference's rank test."""
+        st = np.zeros(6, np.int64)
+        _lib.lib().sprg_mesh_last_stats(self.handle, st.ctypes.data, 6)
+        return dict(zip(["patches", "queued_exact", "expanded", "fallback", "status", "exact_tested"],
+                        st.tolist()))
 
     SHAPE_KEYS = ("fast_threads", "fast_blocks", "exact_blocks", "expand_blocks")
 

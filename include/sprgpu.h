@@ -99,8 +99,9 @@ int sprg_mesh_counts(sprg_mesh* mesh, int64_t* counts);
 int sprg_mesh_phase_times(sprg_mesh* mesh, double* ms, int n);
 /* Patch routes of the last recovery on the mesh: [0] patches, [1] patches
  * queued for the exact rank test / expansion, [2] expanded sets fitted,
- * [3] patches refitted by the fast forms' FP64 fallback, [4] status bits.
- * Writes min(n, 5) values and returns 5.  No reference counterpart. */
+ * [3] patches refitted by the fast forms' FP64 fallback, [4] status bits,
+ * [5] queued patches whose expansion needed the reference's rank test.
+ * Writes min(n, 6) values and returns 6.  No reference counterpart. */
 int sprg_mesh_last_stats(sprg_mesh* mesh, int64_t* stats, int n);
 /* Chooses the launch shapes of the patch kernels for this mesh by timing
  * recoveries of `ip` in `form` (coordinate descent over block sizes and
