@@ -93,6 +93,23 @@ struct FitArgs {
   Lists L;
 };
 
+// The dispatcher deals the workgroups of a launch round-robin over the
+// eight XCDs, each with its own L2: physical block b runs on XCD b % 8, so
+// neighbouring blocks -- which read the same mesh nodes -- land on different
+// L2s.  xcd_block(b) renumbers the blocks so that each XCD walks chunks of
+// K consecutive logical blocks, the chunks of the eight XCDs interleaved
+// over the grid (every XCD still sees work from the whole index range).  A
+// bijection on [0, nblocks); the tail past a multiple of 8K keeps its
+// number; K = 1 is the identity.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks, int K)
+{
+  const int64_t span = 8 * (int64_t)K;
+  const int64_t full = nblocks / span * span;
+  if (b >= full) return b;
+  const int64_t i = b / 8, x = b % 8;
+  return (i / K) * span + x * K + (i % K);
+}
+
 // Appends v to list[*counter] with one atomic per wave and target list.
 // Every lane of the wave calls it (convergent), so the ballot sees the whole
 // wave.

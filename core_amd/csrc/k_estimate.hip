@@ -20,6 +20,8 @@ namespace sprg {
 namespace {
 
 constexpr int EST_T = 256;
+constexpr int XCD_K_ELEM = 64;  // xcd_block chunk of the element pass
+constexpr int XCD_K_AVG = 16;   // and of the vertex average
 
 __device__ __forceinline__ int64_t node_id(const MeshView& m, int64_t t, int n)
 {
@@ -31,7 +33,9 @@ __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
 {
   __shared__ double red[2][EST_T];
   const MeshView& m = a.m;
-  const int64_t t = blockIdx.x * (int64_t)EST_T + threadIdx.x;
+  // logical block: its tets and its slot in the fixed-order partial sums
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x, XCD_K_ELEM);
+  const int64_t t = blk * EST_T + threadIdx.x;
   double self = 0, glob = 0;
   if (t < m.nt) {
     const int nn = tet_nodes(m.order);
@@ -108,8 +112,8 @@ __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    a.partial[2 * blockIdx.x] = red[0][0];
-    a.partial[2 * blockIdx.x + 1] = red[1][0];
+    a.partial[2 * blk] = red[0][0];
+    a.partial[2 * blk + 1] = red[1][0];
   }
 }
 
@@ -158,7 +162,7 @@ __global__ __launch_bounds__(EST_T) void k_factor(EstArgs a)
 __global__ void k_average(EstArgs a)
 {
   const MeshView& m = a.m;
-  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t v = xcd_block(blockIdx.x, gridDim.x, XCD_K_AVG) * blockDim.x + threadIdx.x;
   if (v >= m.nv) return;
   const double sf = a.scalars[2];
   const int32_t b = m.v2t_off[v], e = m.v2t_off[v + 1];

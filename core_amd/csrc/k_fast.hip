@@ -37,6 +37,7 @@ namespace sprg {
 namespace {
 
 constexpr int NQ_MAX = 4;  // points per element the fast kernels handle
+constexpr int XCD_K_FIT = 16;  // xcd_block chunk of the fit pass
 
 template <bool P2MAP>
 struct Geo {
@@ -193,7 +194,8 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // The trip count is the same for every lane of a block, so the list
   // appends below run on whole waves; lanes past the end carry ACT_NONE.
-  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < count; base += stride) {
+  for (int64_t base = xcd_block(blockIdx.x, gridDim.x, XCD_K_FIT) * blockDim.x; base < count;
+       base += stride) {
     const int64_t it = base + threadIdx.x;
     const bool valid = it < count;
     const int32_t entry = !valid ? 0 : src == SRC_REGULAR ? (int32_t)it : list[it];
