@@ -238,6 +238,7 @@ def main():
         return (t_end - t_start) * 1e3 / steps, phases / steps
 
     ms_per_step, phases = timed_steps(args.form, args.warmup, args.steps)
+    routes = m.last_stats()
     log("%s: timed %d steps: %.3f ms/step; phases ms %s" % (
         args.form, args.steps, ms_per_step, np.array2string(phases, precision=3)))
     T, V, E = host.nt, host.nv, host.ne
@@ -268,7 +269,7 @@ def main():
                    "launch_shape": shape or "default"},
         "phases_ms": {"fit": phases[0], "expand": phases[1], "expanded_fit": phases[2],
                       "element": phases[3], "factor_average": phases[4]},
-        "routes": m.last_stats(),
+        "routes": routes,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(args.form),
                      "measured_copy_gbs": probe["gbs"],
@@ -290,14 +291,13 @@ def main():
         forms = {}
         for form in ["fp64f", "fp32r", "fp64"]:
             if form == args.form:
-                ms, ph = ms_per_step, phases
-            elif form == "fp64":
-                ms, ph = timed_steps(form, 1, max(1, args.exact_steps))
+                ms, ph, rt = ms_per_step, phases, routes
             else:
-                ms, ph = timed_steps(form, 1, args.steps)
+                ms, ph = timed_steps(form, 1, max(1, args.exact_steps) if form == "fp64" else args.steps)
+                rt = m.last_stats()
             rec = {"ms_per_step": ms, "ms_median": float(np.median(per_step[form])),
                    "patches_per_s": patches / (ms / 1e3),
-                   "phases_ms": [float(x) for x in ph], "routes": m.last_stats()}
+                   "phases_ms": [float(x) for x in ph], "routes": rt}
             if form != "fp64":
                 sf = spr.getSPRSizeField(eps, 0.05, form=form)
                 rel = (sf.data[:, 0] / a64 - 1).abs()
