@@ -407,11 +407,14 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
   __shared__ double sV[EXACT ? NT * XMAXR : 1];
   __shared__ double sDot[NT];
   __shared__ double sNorm[2];
+  // the candidate hash during enumeration, the screen's sample points after
+  // it (barriers separate the two uses)
   __shared__ double sD[3 * XMAXR];
+  static_assert(HSIZE * sizeof(int32_t) <= sizeof(double) * 3 * XMAXR, "hash must fit the sample buffer");
+  int32_t* const sHash = reinterpret_cast<int32_t*>(sD);
   __shared__ int32_t sCur[256];
   __shared__ int32_t sOld[XSET];
   __shared__ int32_t sTmp[XSET];
-  __shared__ int32_t sHash[HSIZE];
   __shared__ int sN, sOverflow;
   const int lane = threadIdx.x;
   const MeshView& m = a.m;
