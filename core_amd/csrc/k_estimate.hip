@@ -198,11 +198,23 @@ __global__ void k_average(EstArgs a)
 // tet order.
 __global__ void k_vertex_mean(MeshView m, const double* __restrict__ elem, double* __restrict__ out)
 {
-  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t v = xcd_block(blockIdx.x, gridDim.x, XCD_K_AVG) * blockDim.x + threadIdx.x;
   if (v >= m.nv) return;
   const int32_t b = m.v2t_off[v], e = m.v2t_off[v + 1];
+  // as k_average: loads four tets at a time, the sum in adjacency order
   double s = 0;
-  for (int32_t k = b; k < e; ++k) s += elem[m.v2t[k]];
+  int32_t k = b;
+  for (; k + 4 <= e; k += 4) {
+    int32_t t[4];
+    double x[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t[j] = m.v2t[k + j];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) x[j] = elem[t[j]];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s += x[j];
+  }
+  for (; k < e; ++k) s += elem[m.v2t[k]];
   s /= (double)(e - b);
   out[v] = s;
 }
