@@ -177,8 +177,8 @@ def main():
                     help="run the 1 GiB device copy probe and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU only; runs on several GPUs are not supported "
+              "(each GPU would take one mesh part)", file=sys.stderr)
         return 2
 
     import numpy as np

@@ -3,8 +3,8 @@ equations) against the FP64 oracle.
 
 Recovered fields are compared per component, normalised by the component's
 max |oracle| (components that are identically zero are skipped); size fields
-pointwise (they are strictly positive).  The bars are the north-star ones
-for the fast form: <= 1e-6 relative on the size field; the recovered field
+pointwise (they are strictly positive).  The bars for the fast form:
+<= 1e-6 relative on the size field; the recovered field
 is held to 1e-9 of the component scale.
 """
 import numpy as np

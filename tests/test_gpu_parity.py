@@ -3,7 +3,7 @@
 Bit-exact: upward adjacency, patch sets, integration-point gradients and
 recovered fields.  The size fields differ from the oracle only through the
 summation order of two global sums and of the vertex average, so they are
-held to 1e-13 relative (the north-star bar for the FP64 form is 1e-12).
+held to 1e-13 relative (tighter than the 1e-12 required of the FP64 form).
 """
 import numpy as np
 import pytest
