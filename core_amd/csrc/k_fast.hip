@@ -175,7 +175,10 @@ __device__ __forceinline__ void chol_solve(const T* L, const T* rhs, T* z)
   }
 }
 
-enum { SRC_REGULAR = 0, SRC_EXPANDED = 1, SRC_FALLBACK = 2 };
+// SRC_PRE_FALLBACK: the FP32 form's FP64 pass over its fallback queue before
+// the expansion (it may still send a set to the rank test); SRC_FALLBACK then
+// walks only the entries queued after it.
+enum { SRC_REGULAR = 0, SRC_EXPANDED = 1, SRC_FALLBACK = 2, SRC_PRE_FALLBACK = 3 };
 
 // ACT_EXACT: the exact path decides the rank (and expands); ACT_REFIT: the
 // exact kernel refits the certified set with the reference's arithmetic.
@@ -191,15 +194,17 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
   const int32_t* list = src == SRC_EXPANDED ? a.L.xitems[0] : a.L.fbq;
   const int64_t count = src == SRC_REGULAR ? a.npatch
                         : src == SRC_EXPANDED ? a.L.count[NBUCKET] : a.L.count[2 * NBUCKET + 1];
+  const int64_t first = src == SRC_FALLBACK ? a.L.status[2] : 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // The trip count is the same for every lane of a block, so the list
   // appends below run on whole waves; lanes past the end carry ACT_NONE.
-  for (int64_t base = xcd_block(blockIdx.x, gridDim.x, XCD_K_FIT) * blockDim.x; base < count;
+  for (int64_t base = first + xcd_block(blockIdx.x, gridDim.x, XCD_K_FIT) * blockDim.x; base < count;
        base += stride) {
     const int64_t it = base + threadIdx.x;
     const bool valid = it < count;
     const int32_t entry = !valid ? 0 : src == SRC_REGULAR ? (int32_t)it : list[it];
-    const bool expanded = src == SRC_EXPANDED || (src == SRC_FALLBACK && entry < 0);
+    const bool expanded =
+        src == SRC_EXPANDED || ((src == SRC_FALLBACK || src == SRC_PRE_FALLBACK) && entry < 0);
     const int32_t id = entry & 0x7fffffff;
     int64_t p = 0;
     const int32_t* set = nullptr;
@@ -277,15 +282,17 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
     const int bucket = bucket_of(ns * q);
     const int poor = (src == SRC_REGULAR && sizeof(T) == 4) ? ACT_FALLBACK
                      : bucket < NBUCKET                     ? ACT_REFIT
-                     : src != SRC_FALLBACK                  ? ACT_FALLBACK
+                     : src != SRC_FALLBACK && src != SRC_PRE_FALLBACK ? ACT_FALLBACK
                      // more rows than the exact kernel takes: the refined
                      // FP64 solve, unless hopeless
                      : min_ratio >= 1e-13f ? ACT_SOLVE : ACT_ILLCOND;
     int act = ACT_NONE;
     if (short_set)
       act = ACT_EXACT;
-    else if (work && src == SRC_REGULAR && !rank_ok)
-      act = ACT_EXACT;
+    else if (work && (src == SRC_REGULAR || src == SRC_PRE_FALLBACK) && !rank_ok)
+      // the FP32 pass leaves the rank to its FP64 pass, which certifies
+      // most sets the FP32 certificate could not
+      act = (src == SRC_REGULAR && sizeof(T) == 4) ? ACT_FALLBACK : ACT_EXACT;
     else if (work && !(min_ratio >= tau_acc))
       act = poor;
     else if (work)
@@ -415,6 +422,7 @@ int launch_fast_fit(const FitArgs& a, int nterms, int form, int src, hipStream_t
   // exact kernel.  Expanded sets (often barely full rank) always take the
   // FP64 pass.
   if (src == 2) return launch_t<double, 2>(a, nterms, src, 0.0f, 1e-6f, s);
+  if (src == 3) return launch_t<double, 2>(a, nterms, src, 1e-6f, 1e-6f, s);
   if (src == 1) return launch_t<double, 0>(a, nterms, src, 0.0f, 1e-6f, s);
   // FP32: three FP64 refinement steps; each contracts the error by about
   // kappa(G) * 2^-24, and the size field needs the recovered field to ~1e-12

@@ -374,8 +374,8 @@ int setup_lists(sprg_mesh* m, FitArgs& a)
   L.x_cursor = x_cursor;
   L.count = p;
   p += 2 * NBUCKET + 2;
-  L.status = p;
-  p += 2;
+  L.status = p;  // [0] status bits, [1] exact-expansion count, [2] fallback entries taken before the expansion
+  p += 4;
   for (int b = 0; b < NBUCKET; ++b) { L.items[b] = p; p += m->list_cap; }
   for (int b = 0; b < NBUCKET; ++b) { L.xitems[b] = p; p += m->list_cap; }
   L.xq = p; p += m->list_cap;
@@ -422,8 +422,8 @@ int enqueue_recover(sprg_mesh* m, const FitArgs& a, int form, hipStream_t s, boo
   const bool fast = form != SPRG_FP64;
   const int nterms = m->order == 1 ? 4 : 10;
   int rc;
-  // counters, status and the exact-expansion count of the work lists
-  if ((rc = launch_copy_i32(a.L.count, nullptr, 2 * NBUCKET + 4, s))) return rc;
+  // counters, status, the exact-expansion count and the fallback start
+  if ((rc = launch_copy_i32(a.L.count, nullptr, 2 * NBUCKET + 5, s))) return rc;
   if ((rc = launch_copy_i32(reinterpret_cast<int32_t*>(a.L.x_cursor), nullptr, 4, s))) return rc;
   if (marks) {
     m->phase_last = 0;
@@ -431,6 +431,13 @@ int enqueue_recover(sprg_mesh* m, const FitArgs& a, int form, hipStream_t s, boo
   }
   if (fast) {
     if ((rc = launch_fast_fit(a, nterms, form, 0, s))) return rc;
+    if (form == SPRG_FP32R) {
+      // the FP64 pass over the FP32 pass's fallback queue, before the
+      // expansion so that sets it cannot certify still reach the rank test;
+      // the later fallback pass starts after these entries
+      if ((rc = launch_fast_fit(a, nterms, form, 3, s))) return rc;
+      if ((rc = launch_copy_i32(a.L.status + 2, a.L.count + 2 * NBUCKET + 1, 1, s))) return rc;
+    }
   } else {
     if ((rc = launch_classify(a, nterms, s))) return rc;
     if ((rc = launch_exact_fit(a, nterms, 0, s))) return rc;
