@@ -83,15 +83,15 @@ def exact_sequence(ctx, name, h):
         _lib.check(L.sprg_spr_size(m, 2, 9, P(ip), 0.05, P(size), form, P(diag)))
         check("%s size field form %d" % (name, form), np.max(np.abs(size / ref - 1)) <= tol)
         sizes[form] = size
-    # launch shapes chosen by timing change no result (the fast form's tuner
-    # covers the fast-fit, exact-refit and expansion grids)
-    for form in (1,):
-        sh = np.zeros(4, np.int64)
-        _lib.check(L.sprg_mesh_autotune(m, 2, 9, P(ip), form, P(sh), 4))
+    # other launch shapes (small odd grids: every persistent loop wraps)
+    # change no result
+    for form, shape in ((1, [64, 3, 5, 7]), (0, [128, 2, 3, 1])):
+        sh = np.array(shape, np.int64)
+        _lib.check(L.sprg_mesh_set_shape(m, P(sh), 4))
         size = np.full(h.nv, np.nan)
         diag = np.zeros(4)
         _lib.check(L.sprg_spr_size(m, 2, 9, P(ip), 0.05, P(size), form, P(diag)))
-        check("%s autotuned form %d %s" % (name, form, sh.tolist()), np.array_equal(size, sizes[form]))
+        check("%s form %d with launch shape %s" % (name, form, shape), np.array_equal(size, sizes[form]))
     L.sprg_mesh_destroy(m)
 
 
