@@ -173,6 +173,8 @@ def main():
                     help="choose the patch kernels' launch shapes by timing before the timed steps")
     ap.add_argument("--tune-cache", default=None,
                     help="JSON cache of tuned launch shapes (read, and written after tuning)")
+    ap.add_argument("--tile-blocks", type=int, default=None,
+                    help="grid cap of the tiled regular fit pass (0: thread per patch instead)")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run the 1 GiB device copy probe and exit")
     args = ap.parse_args()
@@ -209,6 +211,10 @@ def main():
     torch.cuda.synchronize()
     log("uploaded (%.1fs)" % (time.time() - t0))
     shape = None
+    if args.tile_blocks is not None:
+        shape = {"fast_threads": 256, "fast_blocks": 8192, "exact_blocks": 8192,
+                 "expand_blocks": 4096, "tile_blocks": args.tile_blocks}
+        m.set_shape(shape)
     if args.autotune:
         t1 = time.time()
         shape = m.autotune(eps, args.form, cache=args.tune_cache)

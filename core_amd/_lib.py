@@ -51,8 +51,8 @@ SIGNATURES = {
     "sprg_patch_sets": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp]),
     "sprg_recover": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int]),
     "sprg_spr_size": (_c_int, [_vp, _c_int, _c_int, _vp, _c_dbl, _vp, _c_int, _vp]),
-    "sprg_target_spr_size": (_c_int, [_vp, _c_int, _c_int, _vp, _c_dbl, _c_dbl, _c_dbl, _vp,
-                                      _c_int, _vp]),
+    "sprg_target_spr_size": (_c_int, [_vp, _c_int, _c_int, _vp, ctypes.c_uint64, _c_dbl, _c_dbl,
+                                      _vp, _c_int, _vp]),
     "sprg_average_to_vertex": (_c_int, [_vp, _vp, _vp]),
     "sprg_batch_create": (_c_int, [_vp, _c_int, _vp, _pp]),
     "sprg_batch_destroy": (_c_int, [_vp]),

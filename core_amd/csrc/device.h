@@ -79,11 +79,39 @@ struct LaunchShape {
   int fast_blocks = 8192;   // k_fast persistent grid cap
   int exact_blocks = 8192;  // k_exact_fit grid per bucket (a quarter for 192 rows)
   int expand_blocks = 4096; // k_expand grid
+  int tile_blocks = 8192;   // k_tile persistent grid cap; 0: the regular pass
+                            // runs thread per patch (k_fast) instead
 };
+
+// Tiles of the regular fit pass (built once per mesh, k_tile.hip).  A tile
+// is a run of consecutive vertices with the edges they own (an edge belongs
+// to its lower vertex; its tets are among that vertex's tets), so the union
+// of the tile's vertex patches holds every initial set of the tile.  Each
+// patch is fitted by a group of g = 1, 2, 4 or 8 lanes (by set size), and
+// the groups of a tile are laid out in descending size from lane 0, so a
+// group never straddles a wave and starts at a multiple of its size.
+struct TileView {
+  int64_t ntile = 0;
+  int max_union = 0;              // tets in the largest tile union
+  const int32_t* slot_off = nullptr;  // [ntile+1] lanes of each tile
+  const int32_t* slot = nullptr;      // patch | log2(g) << 27 | rank << 29
+  const int32_t* tet_off = nullptr;   // [ntile+1]
+  const int32_t* tets = nullptr;      // tile unions, ascending tet ids
+  const uint16_t* v2t_loc = nullptr;  // [4nt]: position of v2t[k] in its tile's union
+  const uint16_t* e2t_loc = nullptr;  // [6nt]: the same for e2t
+  const int32_t* leftover = nullptr;  // patches outside every tile
+  int64_t nleftover = 0;
+};
+constexpr int TILE_B = 64;           // lanes per tile: one wave, so a tile's
+                                     // barriers never wait on other waves
+// tets a tile union may hold: the staged samples of 4 points per tet
+// (96 B) and the tet id (4 B) stay within 64 KB of LDS
+constexpr int TILE_MAX_UNION = 640;
 
 struct FitArgs {
   LaunchShape shape;
   MeshView m;
+  TileView tiles;
   int fast;        // fast forms: one regular list, one expanded list
   int q;           // points per element
   int nc;          // components
@@ -140,6 +168,17 @@ int launch_classify(const FitArgs& a, int nterms, hipStream_t s);
 int launch_exact_fit(const FitArgs& a, int nterms, int mode, hipStream_t s);
 int launch_expand(const FitArgs& a, int nterms, hipStream_t s);
 int launch_fast_fit(const FitArgs& a, int nterms, int form, int src, hipStream_t s);
+// The regular pass of the fast forms over the mesh's tiles (k_tile.hip).
+int launch_tile_fit(const FitArgs& a, int nterms, int form, hipStream_t s);
+// Tile construction: tile_vert [2 ntile] (the vertex run [begin, end) of
+// each tile), own_off/own_e (edges owned by each vertex, CSR; null for an
+// order-1 mesh); fills tet_off (device, [ntile+1]), tets and the loc arrays.
+// pass 0 writes each tile's union size to tet_off[tile + 1], pass 1 (after
+// the scan) the unions and positions.
+int launch_tile_build(const MeshView& m, int64_t ntile, const int32_t* tile_vert,
+                      const int32_t* own_off, const int32_t* own_e, int pass, int32_t* tet_off,
+                      int32_t* tets, uint16_t* v2t_loc, uint16_t* e2t_loc, int32_t* status,
+                      hipStream_t s);
 int launch_grad(const MeshView& m, int field_order, int nc_in,
                 const double* nodal, int ip_order, double* ip, hipStream_t s);
 

@@ -1,6 +1,9 @@
 // k_estimate.hip -- the ZZ estimators behind spr::getSPRSizeField
 // (spr/sprEstimateError.cc:98-321) and spr::getTargetSPRSizeField
-// (spr/sprEstimateTargetError.cc:74-258).
+// (spr/sprEstimateTargetError.cc:74-258): SelfProduct (:98-123),
+// ElementError / Error / Error2 (:128-203), computeSizeFactor (:205-217),
+// getCurrentSize (:219-228), getDesiredSize / getElementSizeField
+// (:230-253), averageToVertex / averageSizeField (:255-299).
 //
 // The reference walks the elements three times (SelfProduct, Error, then
 // Error2 per element inside getElementSizeField) and evaluates the recovered
@@ -28,9 +31,32 @@ __device__ __forceinline__ int64_t node_id(const MeshView& m, int64_t t, int n)
   return n < 4 ? (int64_t)m.tv[4 * t + n] : m.nv + (int64_t)m.te[6 * t + (n - 4)];
 }
 
-template <int NC>
+// One thread per tet, the mesh order, component count and IP order fixed
+// at compile time (shape values and rule points fold to constants).
+//
+//   - Recovered values at the points, v2[p][c] = (0 + r_0 N_0(xi_p)) +
+//     r_1 N_1(xi_p) + ... in node order (Element::getComponents): nodes
+//     outer, points inner, so each node's values are loaded once and every
+//     point's sum keeps the reference's order.  (The element error measures
+//     eps - eps*, about 1e-5 of eps, so these sums must round as the
+//     reference's do.)
+//   - dV = det J per point from the coordinate field's gradients in the
+//     reference's order (apf/apfVectorElement.cc:44-66).  For a Lagrange-2
+//     element the per-point sums cancel (terms ~|x|, result ~h), and the
+//     determinant of a poorly shaped tet amplifies that rounding: forming J
+//     once from the vertices instead moved s_e by up to 3e-10 relative on
+//     the jittered n=120 mesh, so every point keeps the reference's J.  On
+//     an order-1 mesh the gradients are 0 and +-1, and the same sums give
+//     x_{i+1} - x_0 exactly, so J is formed once.
+//   - The coordinates are used first (element size, Jacobians) and are dead
+//     before the recovered values accumulate.
+template <int ORDER, int NC, int IPO>
 __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
 {
+  constexpr int NN = ORDER == 1 ? 4 : 10;
+  constexpr int Q = IPO <= 1 ? 1 : IPO == 2 ? 4 : 5;
+  // nodes outer needs Q x NC accumulators; above 36 the points go outer
+  constexpr bool NODES_OUTER = Q * NC <= 36;
   __shared__ double red[2][EST_T];
   const MeshView& m = a.m;
   // logical block: its tets and its slot in the fixed-order partial sums
@@ -38,57 +64,121 @@ __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
   const int64_t t = blk * EST_T + threadIdx.x;
   double self = 0, glob = 0;
   if (t < m.nt) {
-    const int nn = tet_nodes(m.order);
-    const int q = rule_npoints(a.ip_order);
-    double sum = 0;  // ElementError::sum
-    for (int p = 0; p < q; ++p) {
-      double xi[3];
-      rule_point(a.ip_order, p, xi);
-      const double w = rule_weight(a.ip_order, p);
-      double N[10];
-      tet_values(m.order, xi, N);
-      double J[9], v2[NC];
-      for (int n = 0; n < nn; ++n) {
-        double g[3];
-        tet_grad(m.order, xi, n, g);
-        const int64_t nid = node_id(m, t, n);
-        const double* x = m.xyz + 3 * nid;
-        for (int i = 0; i < 3; ++i)
-          for (int j = 0; j < 3; ++j) {
-            double v = x[j] * g[i];
-            J[3 * i + j] = n == 0 ? v : J[3 * i + j] + v;
-          }
-        // Element::getComponents: c = 0; c += data*N in node order
-        const double* r = a.rec + nid * NC;
-        for (int c = 0; c < NC; ++c) {
-          double v = r[c] * N[n];
-          v2[c] = n == 0 ? 0 + v : v2[c] + v;
-        }
-      }
-      const double dV = det3(J);
-      // SelfProduct::atPoint: r += (v*v)*w*dV
-      double vv = 0;
-      for (int c = 0; c < NC; ++c) vv += v2[c] * v2[c];
-      self += vv * w * dV;
-      // ElementError::atPoint: diff = v1 - v2; sum += (diff*diff)*w*dV
-      const double* v1 = a.ip + (t * q + p) * NC;
-      double dd = 0;
-      for (int c = 0; c < NC; ++c) {
-        double d = v1[c] - v2[c];
-        dd += d * d;
-      }
-      sum += dd * w * dV;
+    int64_t nid[NN];
+    {
+      const int4 vv = *reinterpret_cast<const int4*>(m.tv + 4 * t);
+      nid[0] = vv.x; nid[1] = vv.y; nid[2] = vv.z; nid[3] = vv.w;
+      if (ORDER == 2)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nid[4 + (k < 6 ? k : 0)] = m.nv + (int64_t)m.te[6 * t + k];
     }
-    // getCurrentSize: max edge measure (spr/sprEstimateError.cc:286-295)
+    double X[NN][3];
+#pragma unroll
+    for (int n = 0; n < NN; ++n)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) X[n][j] = m.xyz[3 * nid[n] + j];
+    // getCurrentSize: max edge measure (spr/sprEstimateError.cc:219-228)
     double h = 0;
+#pragma unroll
     for (int k = 0; k < 6; ++k) {
-      const double* x0 = m.xyz + 3 * (int64_t)m.tv[4 * t + tet_edge_vert(k, 0)];
-      const double* x1 = m.xyz + 3 * (int64_t)m.tv[4 * t + tet_edge_vert(k, 1)];
-      const double* xm = m.order == 2 ? m.xyz + 3 * (m.nv + (int64_t)m.te[6 * t + k]) : x0;
-      double e = edge_measure(m.order, x0, x1, xm);
+      const double* x0 = X[tet_edge_vert(k, 0)];
+      const double* x1 = X[tet_edge_vert(k, 1)];
+      const double* xm = ORDER == 2 ? X[4 + (ORDER == 2 ? k : 0)] : x0;
+      const double e = edge_measure(ORDER, x0, x1, xm);
       h = (h < e) ? e : h;
     }
-    const double d = 3, po = m.order;
+    double dV[Q];
+    if constexpr (ORDER == 1) {
+      double J[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) J[3 * i + j] = X[i + 1][j] - X[0][j];
+      const double d = det3(J);
+#pragma unroll
+      for (int p = 0; p < Q; ++p) dV[p] = d;
+    } else {
+#pragma unroll 1
+      for (int p = 0; p < Q; ++p) {
+        double xi[3], J[9];
+        rule_point(IPO, p, xi);
+#pragma unroll
+        for (int n = 0; n < NN; ++n) {
+          double g[3];
+          tet_grad(ORDER, xi, n, g);
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              const double v = X[n][j] * g[i];
+              J[3 * i + j] = n == 0 ? v : J[3 * i + j] + v;
+            }
+        }
+        dV[p] = det3(J);
+      }
+    }
+    double sum = 0;  // ElementError::sum
+    auto point = [&](int p, const double* v2) {
+      const double w = rule_weight(IPO, p);
+      // SelfProduct::atPoint: r += (v*v)*w*dV
+      double vv = 0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) vv += v2[c] * v2[c];
+      self += vv * w * dV[p];
+      // ElementError::atPoint: diff = v1 - v2; sum += (diff*diff)*w*dV
+      const double* v1 = a.ip + (t * Q + p) * NC;
+      double dd = 0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const double d = v1[c] - v2[c];
+        dd += d * d;
+      }
+      sum += dd * w * dV[p];
+    };
+    // the coordinates are dead from here; keep the value loads below it
+    asm volatile("" ::: "memory");
+    if constexpr (NODES_OUTER) {
+      double v2[Q][NC];
+#pragma unroll
+      for (int n = 0; n < NN; ++n) {
+        double r[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) r[c] = a.rec[nid[n] * NC + c];
+        // at most two nodes' values in flight
+        if (n & 1) asm volatile("" ::: "memory");
+#pragma unroll
+        for (int p = 0; p < Q; ++p) {
+          double xi[3], N[NN];
+          rule_point(IPO, p, xi);
+          tet_values(ORDER, xi, N);
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const double v = r[c] * N[n];
+            v2[p][c] = n == 0 ? 0 + v : v2[p][c] + v;
+          }
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < Q; ++p) point(p, v2[p]);
+    } else {
+#pragma unroll 1
+      for (int p = 0; p < Q; ++p) {
+        double xi[3], N[NN], v2[NC];
+        rule_point(IPO, p, xi);
+        tet_values(ORDER, xi, N);
+#pragma unroll 1
+        for (int n = 0; n < NN; ++n) {
+          const double* r = a.rec + nid[n] * NC;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const double v = r[c] * N[n];
+            v2[c] = n == 0 ? 0 + v : v2[c] + v;
+          }
+        }
+        point(p, v2);
+      }
+    }
+    const double d = 3, po = ORDER;
     glob = pow(sqrt(sum), ((2 * d) / (2 * po + d)));
     if (!a.target) {
       // Error2 + getDesiredSize: h * errorNorm, size_factor applied later
@@ -96,7 +186,7 @@ __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
       a.elem[t] = h * errorNorm;
     } else {
       // getNewSize: r = pow(theta_e, -2/(2p+d)); clamping needs the factor
-      const int pi = m.order, di = 3;
+      const int pi = ORDER, di = 3;
       a.elem[t] = pow(sqrt(sum), ((-2.0) / (2.0 * pi + di)));
       a.elem[m.nt + t] = h;
     }
@@ -118,7 +208,7 @@ __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
 }
 
 // Fixed-order reduction of the block partials, then the size factor
-// (computeSizeFactor, spr/sprEstimateError.cc:272-284;
+// (computeSizeFactor, spr/sprEstimateError.cc:205-217;
 // target::computeSizeFactor, spr/sprEstimateTargetError.cc:129-137).
 __global__ __launch_bounds__(EST_T) void k_factor(EstArgs a)
 {
@@ -157,7 +247,7 @@ __global__ __launch_bounds__(EST_T) void k_factor(EstArgs a)
   }
 }
 
-// averageToVertex / avgToVtx (spr/sprEstimateError.cc:322-332,
+// averageToVertex / avgToVtx (spr/sprEstimateError.cc:255-265,
 // spr/sprEstimateTargetError.cc:180-190), summing in ascending tet order.
 __global__ void k_average(EstArgs a)
 {
@@ -219,6 +309,35 @@ __global__ void k_vertex_mean(MeshView m, const double* __restrict__ elem, doubl
   out[v] = s;
 }
 
+template <int ORDER, int NC>
+void launch_element(EstArgs& a, hipStream_t s)
+{
+  switch (a.ip_order) {
+    case 1: k_element<ORDER, NC, 1><<<a.nblocks, EST_T, 0, s>>>(a); break;
+    case 2: k_element<ORDER, NC, 2><<<a.nblocks, EST_T, 0, s>>>(a); break;
+    default:
+      // the order-3 rule has 5 points; an order-2 mesh's rule has 4, so only
+      // order-1 meshes take it (check_ip rejects the rest)
+      if constexpr (ORDER == 1) k_element<ORDER, NC, 3><<<a.nblocks, EST_T, 0, s>>>(a);
+      break;
+  }
+}
+
+template <int ORDER>
+int launch_element_ipo(EstArgs& a, hipStream_t s)
+{
+  if (a.ip_order < 1 || a.ip_order > (ORDER == 1 ? 3 : 2))
+    return sprg_fail(SPRG_EUNSUPPORTED, "estimate: integration order %d on an order-%d mesh", a.ip_order, ORDER);
+  switch (a.nc) {
+    case 1: launch_element<ORDER, 1>(a, s); break;
+    case 3: launch_element<ORDER, 3>(a, s); break;
+    case 9: launch_element<ORDER, 9>(a, s); break;
+    case 27: launch_element<ORDER, 27>(a, s); break;
+    default: return sprg_fail(SPRG_EUNSUPPORTED, "estimate: %d components", a.nc);
+  }
+  return 0;
+}
+
 }  // namespace
 
 int launch_vertex_mean(const MeshView& m, const double* elem, double* out, hipStream_t s)
@@ -232,13 +351,9 @@ int launch_estimate(EstArgs& a, hipStream_t s, const std::function<void(int)>& m
   if (a.m.nt == 0) return sprg_fail(SPRG_EINVAL, "estimate: empty mesh");
   a.nblocks = (int)((a.m.nt + EST_T - 1) / EST_T);
   mark(3);
-  switch (a.nc) {
-    case 1: k_element<1><<<a.nblocks, EST_T, 0, s>>>(a); break;
-    case 3: k_element<3><<<a.nblocks, EST_T, 0, s>>>(a); break;
-    case 9: k_element<9><<<a.nblocks, EST_T, 0, s>>>(a); break;
-    case 27: k_element<27><<<a.nblocks, EST_T, 0, s>>>(a); break;
-    default: return sprg_fail(SPRG_EUNSUPPORTED, "estimate: %d components", a.nc);
-  }
+  int rc = 0;
+  rc = a.m.order == 1 ? launch_element_ipo<1>(a, s) : launch_element_ipo<2>(a, s);
+  if (rc) return rc;
   mark(4);
   k_factor<<<1, EST_T, 0, s>>>(a);
   k_average<<<(unsigned)((a.m.nv + 255) / 256), 256, 0, s>>>(a);

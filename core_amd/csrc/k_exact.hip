@@ -468,7 +468,9 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
         done = r_recurrence<NT, 64, XMAXR>(sR, sV, sDot, sNorm, mrows, lane, true);
       }
     }
-    for (int round = 0; round < 64 && !done && !failed && !deferred; ++round) {
+    // every round grows the set (or fails) and a set holds at most XSET
+    // tets (or fails), so the loop ends within XSET + 1 rounds
+    for (int round = 0; round <= XSET && !done && !failed && !deferred; ++round) {
       const int nold = sN;
       for (int i = lane; i < nold; i += 64) sOld[i] = sCur[i];
       __syncthreads();
@@ -556,6 +558,9 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
     }
     if (failed && lane == 0)
       atomicOr(a.L.status, sOverflow ? ST_TOO_LARGE : ST_PATCH_FAIL);
+    // not reachable by the bound above; a patch left without a set must not
+    // leave its recovered value unwritten silently
+    if (!done && !failed && !deferred && lane == 0) atomicOr(a.L.status, ST_INTERNAL);
     if (done) {
       __shared__ int64_t sOff;
       const int ns = sN;

@@ -29,6 +29,7 @@
 // sample positions are recomputed in each (affine in the four vertices for a
 // straight-sided mesh, the Lagrange-2 map otherwise).
 #include "device.h"
+#include "fit_common.h"
 #include "spr_math.h"
 
 #include <algorithm>
@@ -133,56 +134,13 @@ __device__ __forceinline__ void sample_d(const MeshView& m, const Geo<P2MAP>& g,
   d[2] = x[2] - xn[2];
 }
 
-template <class S, int NT>
-__device__ __forceinline__ void basis(const double* d, S* f)
-{
-  const S a = (S)d[0], b = (S)d[1], c = (S)d[2];
-  f[0] = 1;
-  f[1] = a;
-  f[2] = b;
-  f[3] = c;
-  if (NT == 10) {
-    f[4] = a * b;
-    f[5] = b * c;
-    f[6] = c * a;
-    f[7] = a * a;
-    f[8] = b * b;
-    f[9] = c * c;
-  }
-}
-
-// packed lower triangle index
-__host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }
-
-template <class T, int NT>
-__device__ __forceinline__ void chol_solve(const T* L, const T* rhs, T* z)
-{
-  T y[NT];
-#pragma unroll
-  for (int k = 0; k < NT; ++k) {
-    T s = rhs[k];
-#pragma unroll
-    for (int j = 0; j < k; ++j) s = fma(-L[tri(k, j)], y[j], s);
-    y[k] = s / L[tri(k, k)];
-  }
-#pragma unroll
-  for (int kk = 0; kk < NT; ++kk) {
-    const int k = NT - 1 - kk;
-    T s = y[k];
-#pragma unroll
-    for (int j = k + 1; j < NT; ++j) s = fma(-L[tri(j, k)], z[j], s);
-    z[k] = s / L[tri(k, k)];
-  }
-}
-
 // SRC_PRE_FALLBACK: the FP32 form's FP64 pass over its fallback queue before
 // the expansion (it may still send a set to the rank test); SRC_FALLBACK then
 // walks only the entries queued after it.
-enum { SRC_REGULAR = 0, SRC_EXPANDED = 1, SRC_FALLBACK = 2, SRC_PRE_FALLBACK = 3 };
+// SRC_LEFTOVER: the initial sets of the patches outside every tile of the
+// tiled pass (k_tile.hip), with the regular pass's decisions.
+enum { SRC_REGULAR = 0, SRC_EXPANDED = 1, SRC_FALLBACK = 2, SRC_PRE_FALLBACK = 3, SRC_LEFTOVER = 4 };
 
-// ACT_EXACT: the exact path decides the rank (and expands); ACT_REFIT: the
-// exact kernel refits the certified set with the reference's arithmetic.
-enum { ACT_SOLVE = 0, ACT_EXACT = 1, ACT_FALLBACK = 2, ACT_ILLCOND = 3, ACT_NONE = 4, ACT_REFIT = 5 };
 
 // src SRC_REGULAR walks every patch id (no list); the others walk a list.
 template <class T, int NT, int NC, int NREF, bool P2MAP>
@@ -191,9 +149,12 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
   const MeshView& m = a.m;
   const int q = a.q;
   constexpr bool p2map = P2MAP;
-  const int32_t* list = src == SRC_EXPANDED ? a.L.xitems[0] : a.L.fbq;
-  const int64_t count = src == SRC_REGULAR ? a.npatch
-                        : src == SRC_EXPANDED ? a.L.count[NBUCKET] : a.L.count[2 * NBUCKET + 1];
+  const bool regular = src == SRC_REGULAR || src == SRC_LEFTOVER;
+  const int32_t* list = src == SRC_EXPANDED ? a.L.xitems[0] : src == SRC_LEFTOVER ? a.tiles.leftover : a.L.fbq;
+  const int64_t count = src == SRC_REGULAR    ? a.npatch
+                        : src == SRC_LEFTOVER ? a.tiles.nleftover
+                        : src == SRC_EXPANDED ? a.L.count[NBUCKET]
+                                              : a.L.count[2 * NBUCKET + 1];
   const int64_t first = src == SRC_FALLBACK ? a.L.status[2] : 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   // The trip count is the same for every lane of a block, so the list
@@ -226,7 +187,7 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
       }
     }
     // fewer samples than terms (hasEnoughPoints): straight to expansion
-    const bool short_set = valid && src == SRC_REGULAR && ns * q < NT;
+    const bool short_set = valid && regular && ns * q < NT;
     const bool work = valid && !short_set;
     constexpr int NG = NT * (NT + 1) / 2;
     T G[NG];
@@ -280,7 +241,7 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
     // reference's Householder arithmetic reproduces its own answer), when the
     // set fits the exact kernel's rows.
     const int bucket = bucket_of(ns * q);
-    const int poor = (src == SRC_REGULAR && sizeof(T) == 4) ? ACT_FALLBACK
+    const int poor = (regular && sizeof(T) == 4) ? ACT_FALLBACK
                      : bucket < NBUCKET                     ? ACT_REFIT
                      : src != SRC_FALLBACK && src != SRC_PRE_FALLBACK ? ACT_FALLBACK
                      // more rows than the exact kernel takes: the refined
@@ -289,10 +250,10 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
     int act = ACT_NONE;
     if (short_set)
       act = ACT_EXACT;
-    else if (work && (src == SRC_REGULAR || src == SRC_PRE_FALLBACK) && !rank_ok)
+    else if (work && (regular || src == SRC_PRE_FALLBACK) && !rank_ok)
       // the FP32 pass leaves the rank to its FP64 pass, which certifies
       // most sets the FP32 certificate could not
-      act = (src == SRC_REGULAR && sizeof(T) == 4) ? ACT_FALLBACK : ACT_EXACT;
+      act = (regular && sizeof(T) == 4) ? ACT_FALLBACK : ACT_EXACT;
     else if (work && !(min_ratio >= tau_acc))
       act = poor;
     else if (work)
@@ -382,7 +343,9 @@ template <class T, int NT, int NREF, bool P2>
 int launch_nc2(const FitArgs& a, int src, float tau_rank, float tau_acc, hipStream_t s)
 {
   const unsigned threads = (unsigned)a.shape.fast_threads;
-  const int64_t work = src == SRC_REGULAR ? a.npatch : a.npatch / 8 + 1;
+  const int64_t work = src == SRC_REGULAR    ? a.npatch
+                       : src == SRC_LEFTOVER ? a.tiles.nleftover + 1
+                                             : a.npatch / 8 + 1;
   const unsigned blocks =
       (unsigned)std::max<int64_t>(1, std::min<int64_t>(a.shape.fast_blocks, (work + threads - 1) / threads));
   switch (a.nc) {

@@ -1,5 +1,6 @@
 // runtime.hip -- the C ABI: contexts, device memory, the mesh mirror and the
 // SPR entry points (include/sprgpu.h).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -59,6 +60,10 @@ struct sprg_mesh {
   int order = 1;
   int straight = 1;
   DBuf tv, te, xyz, v2t_off, v2t, e2t_off, e2t;
+  // tiles of the fast forms' regular pass (k_tile.hip)
+  DBuf t_slot_off, t_slot, t_tet_off, t_tets, t_v2t_loc, t_e2t_loc, t_leftover;
+  int64_t ntile = 0, nleftover = 0;
+  int max_union = 0;
   // per-call workspace
   DBuf lists, xset, rec, elem, partial, scalars, status, setlen;
   int64_t list_cap = 0, xset_cap = 0;
@@ -91,6 +96,21 @@ struct sprg_mesh {
     return m;
   }
   int64_t npatch() const { return nv + (order == 2 ? ne : 0); }
+  TileView tiles() const
+  {
+    TileView t;
+    t.ntile = ntile;
+    t.max_union = max_union;
+    t.slot_off = t_slot_off.as<int32_t>();
+    t.slot = t_slot.as<int32_t>();
+    t.tet_off = t_tet_off.as<int32_t>();
+    t.tets = t_tets.as<int32_t>();
+    t.v2t_loc = t_v2t_loc.as<uint16_t>();
+    t.e2t_loc = t_e2t_loc.as<uint16_t>();
+    t.leftover = t_leftover.as<int32_t>();
+    t.nleftover = nleftover;
+    return t;
+  }
 };
 
 #define HIP_OK(call, what)                                                    \
@@ -179,6 +199,157 @@ extern "C" int sprg_download(sprg_ctx* ctx, void* dst, const void* src, size_t b
 // ---------------------------------------------------------------------------
 // mesh mirror
 
+namespace {
+
+// log2 of the lanes that fit a patch of ns tets in the tiled pass: about
+// six tets per lane, at most eight lanes
+int group_log2(int64_t ns) { return ns <= 6 ? 0 : ns <= 12 ? 1 : ns <= 24 ? 2 : 3; }
+
+// Builds the tiles of the regular fast pass (k_tile.hip) from the host
+// connectivity and the device CSR: the host groups consecutive vertices
+// (with the edges they own, the lower vertex owning an edge) into tiles of
+// at most TILE_B lanes whose candidate lists (sum of vertex valences) fit
+// the build's sort; the device sorts each tile's union and records every
+// patch's positions in it.  Tiles whose union exceeds TILE_MAX_UNION are
+// rebuilt with a smaller candidate cap.  A vertex too large to share a tile
+// with anything (more than TILE_B lanes, or a valence above the cap) stays
+// outside, with its owned edges: the thread-per-patch pass fits those.
+int build_tiles(sprg_mesh* m, const int32_t* tet_verts, const int32_t* tet_edges)
+{
+  const int64_t nv = m->nv, nt = m->nt;
+  const bool edges = m->order == 2;
+  const int64_t ne = edges ? m->ne : 0;
+  if (nv + ne >= (1 << 27)) return 0;  // the slot encoding holds 27-bit patch ids
+  std::vector<int32_t> vcount(nv, 0), ecount(ne, 0), owner(ne, -1);
+  for (int64_t i = 0; i < 4 * nt; ++i) ++vcount[tet_verts[i]];
+  if (edges)
+    for (int64_t t = 0; t < nt; ++t)
+      for (int k = 0; k < 6; ++k) {
+        const int32_t e = tet_edges[6 * t + k];
+        ++ecount[e];
+        if (owner[e] < 0) {
+          const int32_t a = tet_verts[4 * t + tet_edge_vert(k, 0)], b = tet_verts[4 * t + tet_edge_vert(k, 1)];
+          owner[e] = a < b ? a : b;
+        }
+      }
+  std::vector<int32_t> own_off(nv + 1, 0), own_e(ne);
+  for (int64_t e = 0; e < ne; ++e) ++own_off[owner[e] + 1];
+  for (int64_t v = 0; v < nv; ++v) own_off[v + 1] += own_off[v];
+  {
+    std::vector<int32_t> cur(own_off.begin(), own_off.end() - 1);
+    for (int64_t e = 0; e < ne; ++e) own_e[cur[owner[e]]++] = (int32_t)e;
+  }
+  std::vector<int32_t> vslots(nv);
+  for (int64_t v = 0; v < nv; ++v) {
+    int sl = 1 << group_log2(vcount[v]);
+    for (int32_t i = own_off[v]; i < own_off[v + 1]; ++i) sl += 1 << group_log2(ecount[own_e[i]]);
+    vslots[v] = sl;
+  }
+  hipStream_t s = m->ctx->stream;
+  DBuf d_vert, d_own_off, d_own_e, d_status;
+  int rc = 0;
+  if ((rc = d_own_off.ensure(sizeof(int32_t) * (nv + 1))) || (rc = d_own_e.ensure(sizeof(int32_t) * (ne + 1))) ||
+      (rc = d_status.ensure(sizeof(int32_t))))
+    return rc;
+  HIP_OK(hipMemcpyAsync(d_own_off.p, own_off.data(), sizeof(int32_t) * (nv + 1), hipMemcpyHostToDevice, s), "tiles");
+  if (ne) HIP_OK(hipMemcpyAsync(d_own_e.p, own_e.data(), sizeof(int32_t) * ne, hipMemcpyHostToDevice, s), "tiles");
+  HIP_OK(hipMemsetAsync(d_status.p, 0, sizeof(int32_t), s), "tiles");
+  std::vector<int32_t> tv2, counts, leftover;
+  const MeshView mv = m->view();
+  for (int cap = 2048; cap >= 32; cap /= 2) {
+    tv2.clear();
+    leftover.clear();
+    for (int64_t v = 0; v < nv;) {
+      if (vslots[v] > TILE_B || vcount[v] > cap) {  // fitted outside the tiles
+        leftover.push_back((int32_t)v);
+        for (int32_t i = own_off[v]; i < own_off[v + 1]; ++i) leftover.push_back((int32_t)(nv + own_e[i]));
+        ++v;
+        continue;
+      }
+      const int64_t v0 = v;
+      int sl = 0, sum = 0;
+      while (v < nv && vslots[v] <= TILE_B && vcount[v] <= cap && sl + vslots[v] <= TILE_B &&
+             sum + vcount[v] <= cap) {
+        sl += vslots[v];
+        sum += vcount[v];
+        ++v;
+      }
+      tv2.push_back((int32_t)v0);
+      tv2.push_back((int32_t)v);
+    }
+    const int64_t ntile = (int64_t)tv2.size() / 2;
+    counts.assign(ntile + 1, 0);
+    if (!ntile) break;
+    if ((rc = d_vert.ensure(sizeof(int32_t) * 2 * ntile)) || (rc = m->t_tet_off.ensure(sizeof(int32_t) * (ntile + 1))))
+      return rc;
+    HIP_OK(hipMemcpyAsync(d_vert.p, tv2.data(), sizeof(int32_t) * 2 * ntile, hipMemcpyHostToDevice, s), "tiles");
+    if ((rc = launch_tile_build(mv, ntile, d_vert.as<int32_t>(), nullptr, nullptr, 0, m->t_tet_off.as<int32_t>(),
+                                nullptr, nullptr, nullptr, d_status.as<int32_t>(), s)))
+      return rc;
+    HIP_OK(hipMemcpyAsync(counts.data() + 1, m->t_tet_off.as<int32_t>() + 1, sizeof(int32_t) * ntile,
+                          hipMemcpyDeviceToHost, s), "tiles");
+    HIP_OK(hipStreamSynchronize(s), "tiles");
+    int mx = 0;
+    for (int64_t i = 1; i <= ntile; ++i) mx = std::max(mx, counts[i]);
+    if (mx <= TILE_MAX_UNION || cap == 32) {
+      if (mx > TILE_MAX_UNION) {  // give up tiling: every patch fits thread per patch
+        tv2.clear();
+        counts.assign(1, 0);
+        leftover.clear();
+      }
+      m->max_union = mx <= TILE_MAX_UNION ? mx : 0;
+      break;
+    }
+  }
+  const int64_t ntile = (int64_t)tv2.size() / 2;
+  m->ntile = ntile;
+  m->nleftover = 0;
+  if (!ntile) return 0;
+  for (int64_t i = 0; i < ntile; ++i) counts[i + 1] += counts[i];
+  // slots: every tile's patches in descending group size, groups aligned
+  std::vector<int32_t> slot_off(ntile + 1, 0), slot;
+  slot.reserve((size_t)(4 * nv + ne));
+  std::vector<std::pair<int, int32_t>> pat;
+  for (int64_t k = 0; k < ntile; ++k) {
+    pat.clear();
+    for (int32_t v = tv2[2 * k]; v < tv2[2 * k + 1]; ++v) {
+      pat.push_back({group_log2(vcount[v]), v});
+      for (int32_t i = own_off[v]; i < own_off[v + 1]; ++i)
+        pat.push_back({group_log2(ecount[own_e[i]]), (int32_t)(nv + own_e[i])});
+    }
+    std::stable_sort(pat.begin(), pat.end(),
+                     [](const std::pair<int, int32_t>& x, const std::pair<int, int32_t>& y) { return x.first > y.first; });
+    for (const auto& pr : pat)
+      for (int r = 0; r < (1 << pr.first); ++r)
+        slot.push_back((int32_t)((uint32_t)pr.second | ((uint32_t)pr.first << 27) | ((uint32_t)r << 29)));
+    slot_off[k + 1] = (int32_t)slot.size();
+  }
+  if ((rc = m->t_slot_off.ensure(sizeof(int32_t) * (ntile + 1))) || (rc = m->t_slot.ensure(sizeof(int32_t) * slot.size())) ||
+      (rc = m->t_tets.ensure(sizeof(int32_t) * ((size_t)counts[ntile] + 1))) ||
+      (rc = m->t_v2t_loc.ensure(sizeof(uint16_t) * 4 * nt)) ||
+      (rc = m->t_e2t_loc.ensure(sizeof(uint16_t) * (edges ? 6 * nt : 1))) ||
+      (rc = m->t_leftover.ensure(sizeof(int32_t) * (leftover.size() + 1))))
+    return rc;
+  HIP_OK(hipMemcpyAsync(m->t_tet_off.p, counts.data(), sizeof(int32_t) * (ntile + 1), hipMemcpyHostToDevice, s), "tiles");
+  HIP_OK(hipMemcpyAsync(m->t_slot_off.p, slot_off.data(), sizeof(int32_t) * (ntile + 1), hipMemcpyHostToDevice, s), "tiles");
+  HIP_OK(hipMemcpyAsync(m->t_slot.p, slot.data(), sizeof(int32_t) * slot.size(), hipMemcpyHostToDevice, s), "tiles");
+  if (!leftover.empty())
+    HIP_OK(hipMemcpyAsync(m->t_leftover.p, leftover.data(), sizeof(int32_t) * leftover.size(), hipMemcpyHostToDevice, s),
+           "tiles");
+  if ((rc = launch_tile_build(mv, ntile, d_vert.as<int32_t>(), edges ? d_own_off.as<int32_t>() : nullptr,
+                              d_own_e.as<int32_t>(), 1, m->t_tet_off.as<int32_t>(), m->t_tets.as<int32_t>(),
+                              m->t_v2t_loc.as<uint16_t>(), m->t_e2t_loc.as<uint16_t>(), d_status.as<int32_t>(), s)))
+    return rc;
+  int32_t st = 0;
+  HIP_OK(hipMemcpyAsync(&st, d_status.p, sizeof st, hipMemcpyDeviceToHost, s), "tiles");
+  HIP_OK(hipStreamSynchronize(s), "tiles");
+  if (st) return sprg_fail(SPRG_EINVAL, "mesh_create: tet->edge and tet->vertex connectivity disagree");
+  m->nleftover = (int64_t)leftover.size();
+  return 0;
+}
+
+}  // namespace
+
 extern "C" int sprg_mesh_create(sprg_ctx* ctx, int64_t nv, int64_t nt, int64_t ne,
                                 const int32_t* tet_verts, const int32_t* tet_edges,
                                 const double* node_xyz, int order, sprg_mesh** out)
@@ -255,6 +426,7 @@ extern "C" int sprg_mesh_create(sprg_ctx* ctx, int64_t nv, int64_t nt, int64_t n
   }
   if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = sprg_fail(SPRG_EHIP, "mesh_create: sync failed");
   tmp.release();
+  if (!rc) rc = build_tiles(m, tet_verts, tet_edges);
   if (rc) {
     sprg_mesh_destroy(m);
     return rc;
@@ -270,7 +442,8 @@ extern "C" int sprg_mesh_destroy(sprg_mesh* m)
   hipStreamSynchronize(m->ctx->stream);
   DBuf* all[] = {&m->tv, &m->te, &m->xyz, &m->v2t_off, &m->v2t, &m->e2t_off, &m->e2t,
                  &m->lists, &m->xset, &m->rec, &m->elem, &m->partial, &m->scalars,
-                 &m->status, &m->setlen};
+                 &m->status, &m->setlen, &m->t_slot_off, &m->t_slot, &m->t_tet_off,
+                 &m->t_tets, &m->t_v2t_loc, &m->t_e2t_loc, &m->t_leftover};
   for (DBuf* b : all) b->release();
   for (int i = 0; i <= NPHASE; ++i)
     if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
@@ -430,7 +603,13 @@ int enqueue_recover(sprg_mesh* m, const FitArgs& a, int form, hipStream_t s, boo
     m->mark(0);
   }
   if (fast) {
-    if ((rc = launch_fast_fit(a, nterms, form, 0, s))) return rc;
+    if (a.tiles.ntile > 0 && a.shape.tile_blocks > 0) {
+      // the tiled regular pass, then the patches outside every tile
+      if ((rc = launch_tile_fit(a, nterms, form, s))) return rc;
+      if (a.tiles.nleftover > 0 && (rc = launch_fast_fit(a, nterms, form, 4, s))) return rc;
+    } else if ((rc = launch_fast_fit(a, nterms, form, 0, s))) {
+      return rc;
+    }
     if (form == SPRG_FP32R) {
       // the FP64 pass over the FP32 pass's fallback queue, before the
       // expansion so that sets it cannot certify still reach the rank test;
@@ -462,6 +641,7 @@ FitArgs fit_args(sprg_mesh* m, int q, int nc, const double* ip, double* out, int
   FitArgs a;
   a.shape = m->shape;
   a.m = m->view();
+  a.tiles = m->tiles();
   a.fast = form != SPRG_FP64 ? 1 : 0;
   a.q = q;
   a.nc = nc;
@@ -631,16 +811,19 @@ extern "C" int sprg_spr_size(sprg_mesh* m, int ip_order, int nc, const double* i
 // earlier sprg_mesh_autotune results).
 extern "C" int sprg_mesh_set_shape(sprg_mesh* m, const int64_t* shape, int n)
 {
-  if (!m || !shape || n < 4) return sprg_fail(SPRG_EINVAL, "set_shape: needs 4 values");
+  if (!m || !shape || n < 4) return sprg_fail(SPRG_EINVAL, "set_shape: needs 4 or 5 values");
   if (shape[0] != 64 && shape[0] != 128 && shape[0] != 256)
     return sprg_fail(SPRG_EINVAL, "set_shape: fast-fit block size must be 64, 128 or 256");
   for (int i = 1; i < 4; ++i)
     if (shape[i] < 1 || shape[i] > (1 << 20))
       return sprg_fail(SPRG_EINVAL, "set_shape: grid sizes must lie in [1, 2^20]");
+  if (n >= 5 && (shape[4] < 0 || shape[4] > (1 << 20)))
+    return sprg_fail(SPRG_EINVAL, "set_shape: the tiled grid must lie in [0, 2^20]");
   m->shape.fast_threads = (int)shape[0];
   m->shape.fast_blocks = (int)shape[1];
   m->shape.exact_blocks = (int)shape[2];
   m->shape.expand_blocks = (int)shape[3];
+  if (n >= 5) m->shape.tile_blocks = (int)shape[4];
   return SPRG_OK;
 }
 
@@ -677,6 +860,10 @@ extern "C" int sprg_mesh_autotune(sprg_mesh* m, int ip_order, int nc, const doub
   };
   std::vector<Param> params;
   if (fast) {
+    // the tiled pass is kept on or off as it is: the two regular passes sum
+    // the rows of a patch in different orders
+    if (m->shape.tile_blocks > 0 && m->ntile > 0)
+      params.push_back({&LaunchShape::tile_blocks, {1024, 2048, 4096, 8192, 16384}});
     params.push_back({&LaunchShape::fast_threads, {64, 128, 256}});
     params.push_back({&LaunchShape::fast_blocks, {2048, 4096, 8192, 16384, 32768}});
   } else {
@@ -701,9 +888,9 @@ extern "C" int sprg_mesh_autotune(sprg_mesh* m, int ip_order, int nc, const doub
     }
     m->shape.*prm.field = chosen;
   }
-  const int64_t out[4] = {m->shape.fast_threads, m->shape.fast_blocks, m->shape.exact_blocks,
-                          m->shape.expand_blocks};
-  for (int i = 0; shape && i < n && i < 4; ++i) shape[i] = out[i];
+  const int64_t out[5] = {m->shape.fast_threads, m->shape.fast_blocks, m->shape.exact_blocks,
+                          m->shape.expand_blocks, m->shape.tile_blocks};
+  for (int i = 0; shape && i < n && i < 5; ++i) shape[i] = out[i];
   return SPRG_OK;
 }
 
@@ -718,15 +905,17 @@ extern "C" int sprg_average_to_vertex(sprg_mesh* m, const double* elem, double* 
 }
 
 extern "C" int sprg_target_spr_size(sprg_mesh* m, int ip_order, int nc, const double* ip,
-                                    double target, double alpha, double beta, double* size,
+                                    uint64_t target, double alpha, double beta, double* size,
                                     int form, double* diag)
 {
   if (!m || !ip || !size) return sprg_fail(SPRG_EINVAL, "target_spr_size: null argument");
   // PCU_ALWAYS_ASSERT(target > 0), PCU_ALWAYS_ASSERT(alpha < beta)
   // (spr/sprEstimateTargetError.cc:249-250)
-  if (!(target > 0)) return sprg_fail(SPRG_EINVAL, "getTargetSPRSizeField: target must be > 0");
+  if (target == 0) return sprg_fail(SPRG_EINVAL, "getTargetSPRSizeField: target must be > 0");
   if (!(alpha < beta)) return sprg_fail(SPRG_EINVAL, "getTargetSPRSizeField: alpha must be < beta");
-  return run_estimate(m, ip_order, nc, ip, 1, 0, target, alpha, beta, size, form, diag);
+  // double N = e->target_number (spr/sprEstimateTargetError.cc:135): the
+  // same size_t -> double conversion
+  return run_estimate(m, ip_order, nc, ip, 1, 0, (double)target, alpha, beta, size, form, diag);
 }
 
 // ---------------------------------------------------------------------------
@@ -888,8 +1077,10 @@ uint64_t bits(double x)
 }
 
 // Records the pipelines of every mesh of the batch on the context stream.
+// target: NULL (getSPRSizeField for every mesh) or one element count per
+// mesh (getTargetSPRSizeField).
 int record_batch(sprg_batch* b, int ip_order, int nc, const double* const* ip,
-                 const double* target, double ar, double alpha, double beta,
+                 const uint64_t* target, double ar, double alpha, double beta,
                  double* const* size, int form)
 {
   hipStream_t s = b->ctx->stream;
@@ -912,9 +1103,9 @@ int record_batch(sprg_batch* b, int ip_order, int nc, const double* const* ip,
     sprg_mesh* m = b->meshes[i];
     rc = enqueue_recover(m, fa[i], form, s, false);
     if (!rc) {
-      const bool tgt = target[i] > 0;
-      EstArgs e = est_args(m, ip_order, nc, ip[i], tgt ? 1 : 0, ar, tgt ? target[i] : 0, alpha,
-                           beta, size[i]);
+      const bool tgt = target != nullptr;
+      EstArgs e = est_args(m, ip_order, nc, ip[i], tgt ? 1 : 0, ar, tgt ? (double)target[i] : 0,
+                           alpha, beta, size[i]);
       rc = launch_estimate(e, s, [](int) {});
     }
     if (!rc) rc = launch_copy_i32(b->bstat.as<int32_t>() + i, fa[i].L.status, 1, s);
@@ -934,41 +1125,71 @@ int record_batch(sprg_batch* b, int ip_order, int nc, const double* const* ip,
   return 0;
 }
 
+// What a recorded graph depends on beyond the call's arguments: the launch
+// shapes and every device address its kernels were given.  Workspaces only
+// grow, so they are sized for this call first (no-ops when they fit); an
+// eager call that grew one since the recording shows up as a new address.
+int workspace_key(sprg_batch* b, int q, int nc, int form, std::vector<uint64_t>& key)
+{
+  for (sprg_mesh* m : b->meshes) {
+    int rc = estimate_buffers(m, nc);
+    FitArgs a = fit_args(m, q, nc, nullptr, nullptr, form);
+    if (!rc) rc = setup_lists(m, a);
+    if (rc) return rc;
+    const DBuf* bufs[] = {&m->rec, &m->lists, &m->xset, &m->elem, &m->partial, &m->scalars};
+    for (const DBuf* d : bufs) {
+      key.push_back((uint64_t)(uintptr_t)d->p);
+      key.push_back((uint64_t)d->bytes);
+    }
+    key.push_back((uint64_t)m->list_cap);
+    key.push_back((uint64_t)m->xset_cap);
+    key.push_back((uint64_t)m->shape.fast_threads);
+    key.push_back((uint64_t)m->shape.fast_blocks);
+    key.push_back((uint64_t)m->shape.exact_blocks);
+    key.push_back((uint64_t)m->shape.expand_blocks);
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" int sprg_batch_size(sprg_batch* b, int ip_order, int nc, const double* const* ip,
-                               const double* target, double adapt_ratio, double alpha, double beta,
+                               const uint64_t* target, double adapt_ratio, double alpha, double beta,
                                double* const* size, int form, double* ms)
 {
-  if (!b || !ip || !target || !size) return sprg_fail(SPRG_EINVAL, "batch_size: null argument");
+  if (!b || !ip || !size) return sprg_fail(SPRG_EINVAL, "batch_size: null argument");
   if (!valid_form(form)) return sprg_fail(SPRG_EINVAL, "unknown numeric form %d", form);
   if (nc != 1 && nc != 3 && nc != 9 && nc != 27)
     return sprg_fail(SPRG_EUNSUPPORTED, "estimator: %d components", nc);
   const int n = (int)b->meshes.size();
-  bool any_target = false;
+  int q = 0;
   for (int i = 0; i < n; ++i) {
-    int q, rc = check_ip(b->meshes[i], ip_order, nc, &q);
+    int rc = check_ip(b->meshes[i], ip_order, nc, &q);
     if (rc) return rc;
     if (form != SPRG_FP64 && q > 4)
       return sprg_fail(SPRG_EUNSUPPORTED, "the fast forms take at most 4 points per element");
     if (!ip[i] || !size[i]) return sprg_fail(SPRG_EINVAL, "batch_size: null field of mesh %d", i);
-    any_target = any_target || target[i] > 0;
+    // PCU_ALWAYS_ASSERT(target > 0) (spr/sprEstimateTargetError.cc:249)
+    if (target && target[i] == 0)
+      return sprg_fail(SPRG_EINVAL, "getTargetSPRSizeField: target of mesh %d must be > 0", i);
   }
-  if (any_target && !(alpha < beta))
+  if (target && !(alpha < beta))
     return sprg_fail(SPRG_EINVAL, "getTargetSPRSizeField: alpha must be < beta");
   HIP_OK(hipSetDevice(b->ctx->device), "hipSetDevice");
-  std::vector<uint64_t> key = {(uint64_t)ip_order, (uint64_t)nc, (uint64_t)form, bits(adapt_ratio),
-                               bits(alpha), bits(beta)};
-  for (int i = 0; i < n; ++i) {
-    key.push_back((uint64_t)(uintptr_t)ip[i]);
-    key.push_back((uint64_t)(uintptr_t)size[i]);
-    key.push_back(bits(target[i]));
-  }
   hipStream_t s = b->ctx->stream;
   for (int attempt = 0; attempt < 4; ++attempt) {
+    std::vector<uint64_t> key = {(uint64_t)ip_order, (uint64_t)nc, (uint64_t)form, bits(adapt_ratio),
+                                 bits(alpha), bits(beta), target ? 1u : 0u};
+    for (int i = 0; i < n; ++i) {
+      key.push_back((uint64_t)(uintptr_t)ip[i]);
+      key.push_back((uint64_t)(uintptr_t)size[i]);
+      key.push_back(target ? target[i] : 0);
+    }
+    int rc = workspace_key(b, q, nc, form, key);
+    if (rc) return rc;
     if (!b->exec || key != b->key) {
       b->drop();
-      int rc = record_batch(b, ip_order, nc, ip, target, adapt_ratio, alpha, beta, size, form);
+      rc = record_batch(b, ip_order, nc, ip, target, adapt_ratio, alpha, beta, size, form);
       if (rc) return rc;
       b->key = key;
     }
@@ -982,7 +1203,7 @@ extern "C" int sprg_batch_size(sprg_batch* b, int ip_order, int nc, const double
     bool again = false;
     for (int i = 0; i < n; ++i) {
       bool retry;
-      int rc = status_code(b->meshes[i], st[i], &retry);
+      rc = status_code(b->meshes[i], st[i], &retry);
       if (rc) {
         const std::string msg = sprg_last_error();
         return sprg_fail(rc, "batch mesh %d: %s", i, msg.c_str());

@@ -8,7 +8,7 @@ layout (nodal Lagrange [nodes, nc]; integration point [nt, q, nc]):
 
     getGradIPField(f, name, order)                    spr/sprGetGradIPField.cc:15
     recoverField(ip_field)                            spr/sprRecoverField.cc:419
-    getSPRSizeField(f, adapt_ratio)                   spr/sprEstimateError.cc:378
+    getSPRSizeField(f, adapt_ratio)                   spr/sprEstimateError.cc:311
     getTargetSPRSizeField(f, t, alpha=0.25, beta=2.0) spr/sprEstimateTargetError.cc:241
 
 Error behaviour: where the reference aborts (PCU_ALWAYS_ASSERT, apf::fail)
@@ -172,7 +172,7 @@ class Mesh:
         return dict(zip(["patches", "queued_exact", "expanded", "fallback", "status", "exact_tested"],
                         st.tolist()))
 
-    SHAPE_KEYS = ("fast_threads", "fast_blocks", "exact_blocks", "expand_blocks")
+    SHAPE_KEYS = ("fast_threads", "fast_blocks", "exact_blocks", "expand_blocks", "tile_blocks")
 
     def autotune(self, ip_field, form="fp64f", cache=None):
         """Chooses this mesh's patch-kernel launch shapes by timing
@@ -189,10 +189,10 @@ class Mesh:
             if key in table:
                 self.set_shape(table[key])
                 return dict(table[key])
-        sh = np.zeros(4, np.int64)
+        sh = np.zeros(5, np.int64)
         _sync_torch()
         _lib.check(_lib.lib().sprg_mesh_autotune(self.handle, ip_field.order, ip_field.ncomp,
-                                                 _dptr(ip_field.data), _form(form), sh.ctypes.data, 4))
+                                                 _dptr(ip_field.data), _form(form), sh.ctypes.data, 5))
         shape = dict(zip(self.SHAPE_KEYS, sh.tolist()))
         if key is not None:
             table = _read_json(cache)
@@ -203,8 +203,9 @@ class Mesh:
 
     def set_shape(self, shape):
         """Applies launch shapes as autotune() returns them."""
-        sh = np.array([int(shape[k]) for k in self.SHAPE_KEYS], np.int64)
-        _lib.check(_lib.lib().sprg_mesh_set_shape(self.handle, sh.ctypes.data, 4))
+        keys = [k for k in self.SHAPE_KEYS if k in shape]
+        sh = np.array([int(shape[k]) for k in keys], np.int64)
+        _lib.check(_lib.lib().sprg_mesh_set_shape(self.handle, sh.ctypes.data, len(keys)))
 
     def patch_sets(self, ip_order):
         """(off, idx) of the tet set every recovered entity is fitted on."""
@@ -292,9 +293,12 @@ def getSPRSizeField(f, adapt_ratio, form="fp64", diag=None):
 
 
 def getTargetSPRSizeField(f, t, alpha=0.25, beta=2.0, form="fp64", diag=None):
-    """spr::getTargetSPRSizeField: size field aiming at t output elements."""
+    """spr::getTargetSPRSizeField: size field aiming at t output elements
+    (t is the reference's size_t: a positive integer)."""
     if f is None or f.kind != "ip":
         raise SPRError(_lib.SPRG_EINVAL, "getTargetSPRSizeField needs an integration-point field")
+    if int(t) != t or t < 0:
+        raise SPRError(_lib.SPRG_EINVAL, "getTargetSPRSizeField: the target is an element count")
     m = f.mesh
     _check_temp_names(m, f)
     out = m.createLagrangeField("size", 1, 1)
@@ -302,7 +306,7 @@ def getTargetSPRSizeField(f, t, alpha=0.25, beta=2.0, form="fp64", diag=None):
     _sync_torch()
     try:
         _lib.check(_lib.lib().sprg_target_spr_size(m.handle, f.order, f.ncomp, _dptr(f.data),
-                                                   float(t), float(alpha), float(beta),
+                                                   int(t), float(alpha), float(beta),
                                                    _dptr(out.data), _form(form), d.ctypes.data))
     except SPRError:
         m.destroyField(out)
@@ -357,9 +361,9 @@ class Batch:
 
     def size_fields(self, fields, targets=None, adapt_ratio=0.1, alpha=0.25, beta=2.0,
                     form="fp64", outputs=None):
-        """For mesh i: getTargetSPRSizeField(fields[i], targets[i], alpha,
-        beta) when targets[i] > 0, else getSPRSizeField(fields[i],
-        adapt_ratio).  Returns the "size" tensors; pass the previous
+        """For every mesh i: getSPRSizeField(fields[i], adapt_ratio) when
+        targets is None, else getTargetSPRSizeField(fields[i], targets[i],
+        alpha, beta).  Returns the "size" tensors; pass the previous
         `outputs` back to replay the recorded graph."""
         n = len(self.meshes)
         if len(fields) != n:
@@ -371,12 +375,17 @@ class Batch:
                                                  "one order and component count")
         if outputs is None:
             outputs = [m._tensor((m.nv,)) for m in self.meshes]
-        tg = np.asarray([0.0] * n if targets is None else targets, dtype=np.float64)
+        tg = None
+        if targets is not None:
+            if len(targets) != n or any(int(t) != t or t <= 0 for t in targets):
+                raise SPRError(_lib.SPRG_EINVAL, "batch targets: one positive element count per mesh")
+            tg = np.asarray([int(t) for t in targets], dtype=np.uint64)
         ips = (_lib.ctypes.c_void_p * n)(*[f.data.data_ptr() for f in fields])
         outs = (_lib.ctypes.c_void_p * n)(*[o.data_ptr() for o in outputs])
         ms = np.zeros(1)
         _sync_torch()
-        _lib.check(_lib.lib().sprg_batch_size(self.handle, f0.order, f0.ncomp, ips, tg.ctypes.data,
+        _lib.check(_lib.lib().sprg_batch_size(self.handle, f0.order, f0.ncomp, ips,
+                                              None if tg is None else tg.ctypes.data,
                                               float(adapt_ratio), float(alpha), float(beta), outs,
                                               _form(form), ms.ctypes.data))
         self.last_ms = float(ms[0])

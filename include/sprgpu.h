@@ -95,7 +95,7 @@ int sprg_mesh_counts(sprg_mesh* mesh, int64_t* counts);
  * [1] patch expansion, [2] fit of the expanded sets, [3] element pass,
  * [4] size factor + vertex average (0 for phases the call did not run).
  * Writes min(n, 5) values and returns 5.  No reference counterpart (the
- * reference times the whole call, spr/sprEstimateError.cc:380-386). */
+ * reference times the whole call, spr/sprEstimateError.cc:313-319). */
 int sprg_mesh_phase_times(sprg_mesh* mesh, double* ms, int n);
 /* Patch routes of the last recovery on the mesh: [0] patches, [1] patches
  * queued for the exact rank test / expansion, [2] expanded sets fitted,
@@ -106,15 +106,17 @@ int sprg_mesh_last_stats(sprg_mesh* mesh, int64_t* stats, int n);
 /* Chooses the launch shapes of the patch kernels for this mesh by timing
  * recoveries of `ip` in `form` (coordinate descent over block sizes and
  * persistent grid sizes) and keeps the fastest for the mesh's later calls.
- * Every shape computes every patch with the same arithmetic, so results do
- * not change.  Writes min(n, 4) values (fast-fit threads per block, fast-fit
- * grid cap, exact-fit grid, expansion grid) to `shape`, which may be NULL.
- * No reference counterpart. */
+ * Every shape it tries computes every patch with the same arithmetic, so
+ * results do not change.  Writes min(n, 5) values (fast-fit threads per
+ * block, fast-fit grid cap, exact-fit grid, expansion grid, tiled-pass grid
+ * cap) to `shape`, which may be NULL.  No reference counterpart. */
 int sprg_mesh_autotune(sprg_mesh* mesh, int ip_order, int nc, const double* ip,
                        int form, int64_t* shape, int n);
-/* Sets the four launch-shape values sprg_mesh_autotune reports (block size
- * 64/128/256; grids in [1, 2^20]), e.g. from a cache of tuned shapes.  No
- * reference counterpart. */
+/* Sets the launch-shape values sprg_mesh_autotune reports (n = 4 or 5:
+ * block size 64/128/256; grids in [1, 2^20]; the fifth, the tiled regular
+ * pass's grid cap, in [0, 2^20], where 0 runs the fast forms' regular pass
+ * thread per patch instead -- the same fit, its rows summed in another
+ * order), e.g. from a cache of tuned shapes.  No reference counterpart. */
 int sprg_mesh_set_shape(sprg_mesh* mesh, const int64_t* shape, int n);
 /* Device CSR of getAdjacent(entity, 3): dim 0 (vertices) or 1 (edges).
  * off has n+1 entries, idx has off[n]; both stay owned by the mesh. */
@@ -144,16 +146,19 @@ int sprg_patch_sets(sprg_mesh* mesh, int ip_order, int64_t* set_off,
 int sprg_recover(sprg_mesh* mesh, int ip_order, int nc, const double* ip,
                  double* nodal, int form);
 
-/* ---- spr::getSPRSizeField (spr/sprEstimateError.cc:378-388) ------------
+/* ---- spr::getSPRSizeField (spr/sprEstimateError.cc:311-321) ------------
  * ip [nt][q][nc] -> size [nv].  diag (host, optional, 4 doubles):
  * ||eps*||^2, sum of element error terms, size_factor, wall ms. */
 int sprg_spr_size(sprg_mesh* mesh, int ip_order, int nc, const double* ip,
                   double adapt_ratio, double* size, int form, double* diag);
 
 /* ---- spr::getTargetSPRSizeField (spr/sprEstimateTargetError.cc:241-258)
- * diag (host, optional, 4 doubles): G, size_factor, 0, wall ms. */
+ * target: the element count aimed at (size_t t of the reference; 0 is
+ * rejected with SPRG_EINVAL, PCU_ALWAYS_ASSERT(target > 0) at :249), as is
+ * alpha >= beta (:250).  diag (host, optional, 4 doubles): G, size_factor,
+ * 0, wall ms. */
 int sprg_target_spr_size(sprg_mesh* mesh, int ip_order, int nc,
-                         const double* ip, double target, double alpha,
+                         const double* ip, uint64_t target, double alpha,
                          double beta, double* size, int form, double* diag);
 
 /* ---- spr::averageToVertex (spr/sprEstimateError.cc:255-265) -----------
@@ -163,19 +168,21 @@ int sprg_average_to_vertex(sprg_mesh* mesh, const double* elem, double* vert);
 
 /* ---- many meshes per launch sequence ------------------------------------
  * A batch holds meshes of one context (each at most once).  sprg_batch_size
- * runs, for every mesh i, spr::getTargetSPRSizeField(ip[i], target[i],
- * alpha, beta) when target[i] > 0, else spr::getSPRSizeField(ip[i],
- * adapt_ratio), writing size[i] [nv_i] (device pointers).  The first call
- * records all pipelines into one HIP graph; calls with the same fields and
- * parameters replay it (others record again).  ms (optional): wall time of
- * the replay.  Results equal the per-mesh calls bit for bit.  No reference
- * counterpart: the reference estimates one mesh per call. */
+ * runs, for every mesh i, spr::getSPRSizeField(ip[i], adapt_ratio) when
+ * target is NULL, else spr::getTargetSPRSizeField(ip[i], target[i], alpha,
+ * beta) (every target[i] > 0), writing size[i] [nv_i] (device pointers).
+ * The first call records all pipelines into one HIP graph; later calls
+ * replay it when the fields, parameters, launch shapes and every mesh's
+ * workspace are unchanged (otherwise they record again: an eager call that
+ * grows a mesh's workspace invalidates the recorded graph).  ms (optional):
+ * wall time of the replay.  Results equal the per-mesh calls bit for bit.
+ * No reference counterpart: the reference estimates one mesh per call. */
 typedef struct sprg_batch sprg_batch;
 int sprg_batch_create(sprg_ctx* ctx, int nmesh, sprg_mesh* const* meshes,
                       sprg_batch** out);
 int sprg_batch_destroy(sprg_batch* batch);
 int sprg_batch_size(sprg_batch* batch, int ip_order, int nc,
-                    const double* const* ip, const double* target,
+                    const double* const* ip, const uint64_t* target,
                     double adapt_ratio, double alpha, double beta,
                     double* const* size, int form, double* ms);
 

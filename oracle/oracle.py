@@ -168,6 +168,19 @@ def spr_size(mesh, ip_order, ip, adapt_ratio):
     return size, diag
 
 
+def spr_size_rec(mesh, ip_order, ip, rec, adapt_ratio):
+    """getSPRSizeField's estimator stage from a given recovered field."""
+    ip = np.ascontiguousarray(ip, dtype=np.float64)
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    nc = ip.shape[2]
+    size = np.zeros(mesh.nv)
+    diag = np.zeros(4)
+    f = lib().om_spr_size_rec
+    f.argtypes = [_p, ctypes.c_int, ctypes.c_int, _p, _p, ctypes.c_double, _p, _p]
+    _check(f(mesh.c(), ip_order, nc, _ptr(ip), _ptr(rec), adapt_ratio, _ptr(size), _ptr(diag)))
+    return size, diag
+
+
 def target_size(mesh, ip_order, ip, target, alpha=0.25, beta=2.0):
     ip = np.ascontiguousarray(ip, dtype=np.float64)
     nc = ip.shape[2]

@@ -727,7 +727,8 @@ static void gather_rec(const om_mesh* m, int64_t t, int nc, const double* rec,
     for (int c = 0; c < nc; ++c) data[n * nc + c] = rec[nid[n] * nc + c];
 }
 
-/* ElementError::atPoint summed over the rule (:220-228, via
+/* ElementError::atPoint summed over the rule (spr/sprEstimateError.cc:153-160,
+ * spr/sprEstimateTargetError.cc:99-113, via
  * Integrator::process(MeshElement*), apf/apfIntegrate.cc:662-676). */
 static double element_error(const om_mesh* m, int ip_order, int nc,
                             const double* ip, const double* rec, int64_t t)
@@ -799,7 +800,9 @@ static double edge_measure(int order, const double* x0, const double* x1,
   return meas;
 }
 
-/* getCurrentSize (:286-295): max edge measure over the six edges. */
+/* getCurrentSize (spr/sprEstimateError.cc:219-228,
+ * spr/sprEstimateTargetError.cc:139-147): max edge measure over the six
+ * edges. */
 static const int tev[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
 
 static double current_size(const om_mesh* m, int64_t t)
@@ -821,7 +824,9 @@ int om_current_size(const om_mesh* m, double* h)
   return 0;
 }
 
-/* averageToVertex (:322-332): mean of the element sizes around a vertex.
+/* averageToVertex (spr/sprEstimateError.cc:255-265; avgToVtx,
+ * spr/sprEstimateTargetError.cc:180-190): mean of the element sizes around a
+ * vertex.
  * The reference sums in mds get_up order (mds/mds.c:695-713); this
  * restatement sums in ascending tet order, which can differ from it by a few
  * ulp per vertex. */
@@ -844,8 +849,8 @@ static int check_ip(const om_mesh* m, int ip_order)
   return 0;
 }
 
-/* getSPRSizeField (:301-321, :368-388): recover, size factor, element size,
- * vertex average. */
+/* getSPRSizeField (spr/sprEstimateError.cc:301-321): recover, size factor,
+ * element size, vertex average. */
 int om_spr_size_rec(const om_mesh* m, int ip_order, int nc, const double* ip,
                     const double* rec, double adapt_ratio, double* size,
                     double* diag)
@@ -854,7 +859,7 @@ int om_spr_size_rec(const om_mesh* m, int ip_order, int nc, const double* ip,
   if (err) return err;
   int q = om_rule_npoints(ip_order);
   double d = 3, p = m->order;
-  /* SelfProduct (:164-190) */
+  /* SelfProduct (spr/sprEstimateError.cc:98-123) */
   double self = 0;
   int nn = tet_nodes(m->order);
   double* data = (double*)malloc(sizeof(double) * 10 * 9);
@@ -876,7 +881,7 @@ int om_spr_size_rec(const om_mesh* m, int ip_order, int nc, const double* ip,
     }
   }
   free(data);
-  /* Error (:237-252) and computeSizeFactor (:272-284) */
+  /* Error (spr/sprEstimateError.cc:171-185) and computeSizeFactor (:205-217) */
   double* s = (double*)malloc(sizeof(double) * (size_t)m->nt);
   double errsum = 0;
   for (int64_t t = 0; t < m->nt; ++t) {
@@ -887,7 +892,8 @@ int om_spr_size_rec(const om_mesh* m, int ip_order, int nc, const double* ip,
   double a = adapt_ratio * adapt_ratio * epsStarNorm * epsStarNorm;
   double b = a / errsum;
   double size_factor = pow(b, 1.0 / (2.0 * p));
-  /* getElementSizeField (:297-320): Error2 recomputes the same s_e */
+  /* getDesiredSize / getElementSizeField (spr/sprEstimateError.cc:230-253):
+   * Error2 (:189-203) recomputes the same s_e */
   double* esize = (double*)malloc(sizeof(double) * (size_t)m->nt);
   for (int64_t t = 0; t < m->nt; ++t) {
     double errorNorm = pow(sqrt(s[t]), -(2 / (2 * p + d)));
@@ -915,7 +921,9 @@ int om_spr_size(const om_mesh* m, int ip_order, int nc, const double* ip,
   return err;
 }
 
-/* getTargetSPRSizeField (spr/sprEstimateTargetError.cc:129-258). */
+/* getTargetSPRSizeField (spr/sprEstimateTargetError.cc:241-258): GlobalErrorTerm
+ * (:116-127), computeSizeFactor (:129-137), getNewSize (:149-164), vertex
+ * average (:180-227). */
 int om_target_size_rec(const om_mesh* m, int ip_order, int nc,
                        const double* ip, const double* rec, double target,
                        double alpha, double beta, double* size, double* diag)

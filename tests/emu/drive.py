@@ -44,7 +44,7 @@ def device_mesh(ctx, h):
     return m
 
 
-def exact_sequence(ctx, name, h):
+def exact_sequence(ctx, name, h, shapes=True, fast=True):
     om = O.Mesh(h.tv, h.xyz, h.order, te=h.te, ev=h.ev)
     m = device_mesh(ctx, h)
     nodes = h.nv + (h.ne if h.order == 2 else 0)
@@ -77,17 +77,23 @@ def exact_sequence(ctx, name, h):
     check("%s recovered field" % name, np.array_equal(out, O.recover(om, ip)))
     ref, _ = O.spr_size(om, 2, ip, 0.05)
     sizes = {}
-    for form, tol in ((0, 1e-13), (1, 1e-6), (2, 1e-6)):
+    for form, tol in ((0, 1e-13), (1, 1e-6), (2, 1e-6)) if fast else ((0, 1e-13),):
         size = np.full(h.nv, np.nan)
         diag = np.zeros(4)
         _lib.check(L.sprg_spr_size(m, 2, 9, P(ip), 0.05, P(size), form, P(diag)))
         check("%s size field form %d" % (name, form), np.max(np.abs(size / ref - 1)) <= tol)
         sizes[form] = size
+    tref, _ = O.target_size(om, 2, g, 2 * h.nt, 0.25, 2.0)
+    for form, tol in ((0, 1e-13), (2, 1e-6)) if fast else ((0, 1e-13),):
+        size = np.full(h.nv, np.nan)
+        diag = np.zeros(4)
+        _lib.check(L.sprg_target_spr_size(m, 2, 9, P(g), 2 * h.nt, 0.25, 2.0, P(size), form, P(diag)))
+        check("%s target size field form %d" % (name, form), np.max(np.abs(size / tref - 1)) <= tol)
     # other launch shapes (small odd grids: every persistent loop wraps)
     # change no result
-    for form, shape in ((1, [64, 3, 5, 7]), (0, [128, 2, 3, 1])):
+    for form, shape in ((1, [64, 3, 5, 7, 2]), (0, [128, 2, 3, 1, 3])) if shapes else ():
         sh = np.array(shape, np.int64)
-        _lib.check(L.sprg_mesh_set_shape(m, P(sh), 4))
+        _lib.check(L.sprg_mesh_set_shape(m, P(sh), 5))
         size = np.full(h.nv, np.nan)
         diag = np.zeros(4)
         _lib.check(L.sprg_spr_size(m, 2, 9, P(ip), 0.05, P(size), form, P(diag)))
@@ -96,38 +102,57 @@ def exact_sequence(ctx, name, h):
 
 
 def batch_check(ctx):
-    # two P1 meshes, one with the target estimator and one with the
-    # adapt-ratio estimator, in one recorded graph
-    hs, ms, eps, eager, tg = [], [], [], [], []
+    # two P1 meshes in one recorded graph, first with the target estimator,
+    # then with the adapt-ratio estimator; between replays, eager calls
+    # that grow a mesh's workspaces (27 components) must make the batch
+    # record again instead of replaying kernels bound to freed buffers
+    hs, ms, eps = [], [], []
     for i, n in enumerate([2, 3]):
         h = mesh.box(n, order=1, jitter=0.2, seed=workloads.C4 + i)
         m = device_mesh(ctx, h)
         u = np.ascontiguousarray(workloads.sin_field(h.xyz).reshape(-1, 1))
         g = np.full((h.nt, 4, 3), np.nan)
         _lib.check(L.sprg_grad_ip(m, 1, 1, P(u), 2, P(g)))
-        s = np.full(h.nv, np.nan)
-        diag = np.zeros(4)
-        if i == 0:
-            t = 2.0 * h.nt
-            _lib.check(L.sprg_target_spr_size(m, 2, 3, P(g), t, 0.25, 2.0, P(s), 2, P(diag)))
-        else:
-            t = 0.0
-            _lib.check(L.sprg_spr_size(m, 2, 3, P(g), 0.1, P(s), 2, P(diag)))
-        hs.append(h), ms.append(m), eps.append(g), eager.append(s), tg.append(t)
+        hs.append(h), ms.append(m), eps.append(g)
+
+    def eager(targets):
+        out = []
+        for m, h, g, t in zip(ms, hs, eps, targets if targets is not None else [0, 0]):
+            s = np.full(h.nv, np.nan)
+            diag = np.zeros(4)
+            if targets is not None:
+                _lib.check(L.sprg_target_spr_size(m, 2, 3, P(g), t, 0.25, 2.0, P(s), 2, P(diag)))
+            else:
+                _lib.check(L.sprg_spr_size(m, 2, 3, P(g), 0.1, P(s), 2, P(diag)))
+            out.append(s)
+        return out
+
+    def grow_workspaces():
+        for m, h in zip(ms, hs):
+            ip27 = np.ascontiguousarray(np.tile(workloads.strain_field(h.ip_points(2)), (1, 1, 3)))
+            s = np.full(h.nv, np.nan)
+            diag = np.zeros(4)
+            _lib.check(L.sprg_spr_size(m, 2, 27, P(ip27), 0.1, P(s), 2, P(diag)))
+
     arr = (vp * 2)(*[m.value for m in ms])
     b = vp()
     _lib.check(L.sprg_batch_create(ctx, 2, arr, ctypes.byref(b)))
     outs = [np.full(h.nv, -1.0) for h in hs]
     ips = (vp * 2)(*[e.ctypes.data for e in eps])
     ops = (vp * 2)(*[o.ctypes.data for o in outs])
-    tga = np.array(tg)
     t = np.zeros(1)
-    for k in range(2):
-        for o in outs:
-            o[:] = -1.0
-        _lib.check(L.sprg_batch_size(b, 2, 3, ips, P(tga), 0.1, 0.25, 2.0, ops, 2, P(t)))
-        check("batch run %d equals the per-mesh calls" % k,
-              all(np.array_equal(o, e) for o, e in zip(outs, eager)))
+    for targets in ([2 * h.nt for h in hs], None):
+        ref = eager(targets)
+        tga = None if targets is None else np.array(targets, np.uint64)
+        for k in range(3):
+            if k == 2:
+                grow_workspaces()
+            for o in outs:
+                o[:] = -1.0
+            _lib.check(L.sprg_batch_size(b, 2, 3, ips, None if tga is None else P(tga), 0.1, 0.25, 2.0,
+                                         ops, 2, P(t)))
+            check("batch (%s) run %d equals the per-mesh calls" % ("target" if targets else "ratio", k),
+                  all(np.array_equal(o, e) for o, e in zip(outs, ref)))
     L.sprg_batch_destroy(b)
     for m in ms:
         L.sprg_mesh_destroy(m)
@@ -137,6 +162,10 @@ def main():
     ctx = vp()
     _lib.check(L.sprg_ctx_create(0, ctypes.byref(ctx)))
     exact_sequence(ctx, "p1 n=2 jittered", mesh.box(2, order=1, jitter=0.25, seed=11))
+    # the unjittered P1 box at 4 points per tet (every tet's samples collapse
+    # to its centroid: most patches take the rank test and the expansion)
+    # (EMU_FULL=1 adds its fast forms, a few minutes more)
+    exact_sequence(ctx, "p1 n=4", mesh.box(4, order=1), shapes=False, fast=bool(os.environ.get("EMU_FULL")))
     batch_check(ctx)
     print("ALL PASS", flush=True)
 

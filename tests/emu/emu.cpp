@@ -93,24 +93,38 @@ int64_t shfl64(int64_t v, int src)
 struct Rec {
   const char* where;
   dim3 grid, block;
+  size_t shmem;
   std::function<void()> fn;
 };
+
+// dynamic LDS of the running launch: a heap block of exactly the requested
+// size, so that AddressSanitizer reports any access past it
+static void* g_dyn = nullptr;
+static size_t g_dyn_bytes = 0;
+void* dyn_lds() { return g_dyn; }
 static bool g_capturing = false;
 static std::vector<Rec> g_rec;
 
-void run_launch(const char* where, dim3 grid, dim3 block, const std::function<void()>& fn);
+void run_launch(const char* where, dim3 grid, dim3 block, size_t shmem, const std::function<void()>& fn);
 
-void launch(const char* where, dim3 grid, dim3 block, const std::function<void()>& fn)
+void launch(const char* where, dim3 grid, dim3 block, size_t shmem, const std::function<void()>& fn)
 {
   if (g_capturing) {
-    g_rec.push_back({where, grid, block, fn});
+    g_rec.push_back({where, grid, block, shmem, fn});
     return;
   }
-  run_launch(where, grid, block, fn);
+  run_launch(where, grid, block, shmem, fn);
 }
 
-void run_launch(const char* where, dim3 grid, dim3 block, const std::function<void()>& fn)
+void run_launch(const char* where, dim3 grid, dim3 block, size_t shmem, const std::function<void()>& fn)
 {
+  if (shmem > 64 * 1024) {
+    fprintf(stderr, "EMU launch %s: %zu bytes of dynamic LDS exceed 64 KB\n", where, shmem);
+    abort();
+  }
+  free(g_dyn);
+  g_dyn = shmem ? malloc(shmem) : nullptr;
+  g_dyn_bytes = shmem;
   if (block.x == 0 || block.x > 1024 || block.y != 1 || block.z != 1 || grid.y != 1 || grid.z != 1) {
     fprintf(stderr, "EMU bad launch %s: grid %u block %u\n", where, grid.x, block.x);
     abort();
@@ -137,7 +151,10 @@ void run_launch(const char* where, dim3 grid, dim3 block, const std::function<vo
         if (g_poison) {
           // fresh LDS holds garbage: fill every __shared__ variable of the
           // kernels run so far with 0xFF (int -1, double NaN) before a block
-          if (t == 0) lds_poison_all();
+          if (t == 0) {
+            lds_poison_all();
+            if (g_dyn) memset(g_dyn, 0xFF, g_dyn_bytes);
+          }
           g_block.wait("poison");
         }
         fn();
@@ -150,7 +167,7 @@ void run_launch(const char* where, dim3 grid, dim3 block, const std::function<vo
 bool capturing() { return g_capturing; }
 void begin_capture() { g_capturing = true; g_rec.clear(); }
 std::vector<Rec> end_capture() { g_capturing = false; std::vector<Rec> r; r.swap(g_rec); return r; }
-void replay(const std::vector<Rec>& r) { for (auto& x : r) run_launch(x.where, x.grid, x.block, x.fn); }
+void replay(const std::vector<Rec>& r) { for (auto& x : r) run_launch(x.where, x.grid, x.block, x.shmem, x.fn); }
 
 }  // namespace emu
 

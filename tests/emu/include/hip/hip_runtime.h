@@ -29,7 +29,10 @@ extern thread_local dim3 threadIdx, blockIdx;
 extern dim3 blockDim, gridDim;
 
 namespace emu {
-void launch(const char* where, dim3 grid, dim3 block, const std::function<void()>& fn);
+void launch(const char* where, dim3 grid, dim3 block, size_t shmem, const std::function<void()>& fn);
+// the launch's dynamic LDS (extern __shared__), shmem bytes, poisoned like
+// static LDS before each block
+void* dyn_lds();
 void block_sync();
 void lds_reg(void* p, size_t n);
 uint64_t ballot(bool p);
@@ -57,6 +60,53 @@ inline double __shfl_xor(double v, int mask)
   memcpy(&r, &b, 8);
   return r;
 }
+inline float __shfl_xor(float v, int mask)
+{
+  int32_t b;
+  memcpy(&b, &v, 4);
+  b = (int32_t)emu::shfl64(b, (int)((threadIdx.x & 63) ^ mask));
+  float r;
+  memcpy(&r, &b, 4);
+  return r;
+}
+// DPP quad permutation (dpp_ctrl < 0x100) with every row and bank enabled;
+// other controls are not emulated
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound)
+{
+  (void)old;
+  (void)bound;
+  if (ctrl >= 0x100 || row_mask != 0xF || bank_mask != 0xF) {
+    fprintf(stderr, "EMU: DPP control 0x%x not emulated\n", ctrl);
+    abort();
+  }
+  const int l = threadIdx.x & 63;
+  const int from = (l & ~3) | ((ctrl >> (2 * (l & 3))) & 3);
+  return (int)emu::shfl64(src, from);
+}
+// ds_swizzle in bitmask mode (offset bit 15 clear): within groups of 32
+// lanes, lane l reads ((l & and) | or) ^ xor
+inline int __builtin_amdgcn_ds_swizzle(int src, int pattern)
+{
+  if (pattern & 0x8000) {
+    fprintf(stderr, "EMU: swizzle mode 0x%x not emulated\n", pattern);
+    abort();
+  }
+  const int and_m = pattern & 0x1F, or_m = (pattern >> 5) & 0x1F, xor_m = (pattern >> 10) & 0x1F;
+  const int l = threadIdx.x & 63;
+  const int from = (l & 32) | (((l & 31 & and_m) | or_m) ^ xor_m);
+  return (int)emu::shfl64(src, from);
+}
+inline int __double2loint(double x) { uint64_t u; memcpy(&u, &x, 8); return (int)(uint32_t)u; }
+inline int __double2hiint(double x) { uint64_t u; memcpy(&u, &x, 8); return (int)(uint32_t)(u >> 32); }
+inline double __hiloint2double(int hi, int lo)
+{
+  const uint64_t u = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+  double x;
+  memcpy(&x, &u, 8);
+  return x;
+}
+inline int __float_as_int(float x) { int i; memcpy(&i, &x, 4); return i; }
+inline float __int_as_float(int i) { float x; memcpy(&x, &i, 4); return x; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 

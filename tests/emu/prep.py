@@ -65,12 +65,22 @@ def rewrite(text):
             e += 1
         args = text[a0 + 1:e]
         tag = re.sub(r"\s+", " ", name)
-        out += text[i:s] + 'emu::launch("%s", dim3(%s), dim3(%s), [=] { %s(%s); })' % (
-            tag, cfg[0], cfg[1], name, args)
+        shmem = cfg[2] if len(cfg) > 2 else "0"
+        out += text[i:s] + 'emu::launch("%s", dim3(%s), dim3(%s), %s, [=] { %s(%s); })' % (
+            tag, cfg[0], cfg[1], shmem, name, args)
         i = e + 1
 
 
 SHARED = re.compile(r"^(\s*)__shared__\s+([A-Za-z_][\w:]*)\s+([^;]+);", re.M)
+DYN_SHARED = re.compile(r"extern\s+__shared__\s+([A-Za-z_][\w:]*)\s+(\w+)\s*\[\s*\]\s*;")
+# empty inline-asm register fences (scheduling hints for the GPU compiler)
+ASM_FENCE = re.compile(r"^\s*asm volatile\(\"\" : \"\+v\"\([^;]*\)\);.*$", re.M)
+
+
+def dyn_shared(text):
+    text = DYN_SHARED.sub(lambda mo: "%s* const %s = reinterpret_cast<%s*>(emu::dyn_lds());"
+                          % (mo.group(1), mo.group(2), mo.group(1)), text)
+    return ASM_FENCE.sub("", text)
 
 
 def reg_shared(text):
@@ -87,7 +97,7 @@ def main():
     for f in os.listdir(SRC):
         p = os.path.join(SRC, f)
         if f.endswith(".hip") or f.endswith(".cpp"):
-            t = reg_shared(rewrite(open(p).read()))
+            t = reg_shared(dyn_shared(rewrite(open(p).read())))
             t = t.replace('#include "device.h"', '#include "device.h"')
             open(os.path.join(OUT, f.rsplit(".", 1)[0] + ".cpp"), "w").write(t)
         elif f.endswith(".h"):

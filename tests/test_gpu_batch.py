@@ -38,7 +38,7 @@ def test_multi_bump_batch(gpu, form):
     t0 = time.time()
     meshes, fields = _build(sizes)
     print("built %d meshes, %d tets (%.1fs)" % (len(meshes), sum(m.nt for m in meshes), time.time() - t0))
-    targets = [2.0 * m.nt for m in meshes]
+    targets = [2 * m.nt for m in meshes]
     # per-mesh eager calls
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -71,4 +71,34 @@ def test_multi_bump_batch(gpu, form):
             ref, _ = O.target_size(om, 2, eps.numpy(), targets[i], 0.25, 2.0)
             got = outs[i].cpu().numpy()
             assert np.max(np.abs(got / ref - 1)) <= 1e-13, i
+    batch.close()
+
+
+def test_batch_records_again_after_workspace_growth(gpu):
+    """An eager call that grows a batch mesh's workspaces (27 components
+    instead of 3) moves them to new device addresses; the next batch call
+    must record again rather than replay kernels bound to the freed ones,
+    and still equal the per-mesh calls."""
+    meshes, fields = _build([3, 4, 5])
+    targets = [2 * m.nt for m in meshes]
+    eager = []
+    for (h, u, eps), t in zip(fields, targets):
+        s = spr.getTargetSPRSizeField(eps, t, 0.25, 2.0, form="fp64f")
+        eager.append(s.data[:, 0].clone())
+        eps.mesh.destroyField(s)
+    batch = spr.Batch(meshes)
+    outs = batch.size_fields([f[2] for f in fields], targets, form="fp64f")
+    for o, e in zip(outs, eager):
+        assert torch.equal(o, e)
+    for m, (h, u, eps) in zip(meshes, fields):
+        ip27 = np.tile(workloads.strain_field(h.ip_points(2)), (1, 1, 3))
+        f27 = m.createIPField("eps27", 27, 2, ip27)
+        s = spr.getSPRSizeField(f27, 0.1, form="fp64f")
+        m.destroyField(s)
+        m.destroyField(f27)
+    for o in outs:
+        o.fill_(-1.0)
+    outs = batch.size_fields([f[2] for f in fields], targets, form="fp64f", outputs=outs)
+    for o, e in zip(outs, eager):
+        assert torch.equal(o, e)
     batch.close()
