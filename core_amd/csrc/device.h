@@ -94,19 +94,22 @@ struct TileView {
   int64_t ntile = 0;
   int max_union = 0;              // tets in the largest tile union
   const int32_t* slot_off = nullptr;  // [ntile+1] lanes of each tile
-  const int32_t* slot = nullptr;      // patch | log2(g) << 27 | rank << 29
+  // per lane: {patch | log2(g) << 27 | rank << 29, first entry of the
+  // patch's set in loc, set size, 0}
+  const int4* slot = nullptr;
   const int32_t* tet_off = nullptr;   // [ntile+1]
   const int32_t* tets = nullptr;      // tile unions, ascending tet ids
-  const uint16_t* v2t_loc = nullptr;  // [4nt]: position of v2t[k] in its tile's union
-  const uint16_t* e2t_loc = nullptr;  // [6nt]: the same for e2t
+  // [4nt + 6nt]: the position of each v2t entry, then of each e2t entry,
+  // in the union of the tile that fits its patch
+  const uint16_t* loc = nullptr;
   const int32_t* leftover = nullptr;  // patches outside every tile
   int64_t nleftover = 0;
 };
 constexpr int TILE_B = 64;           // lanes per tile: one wave, so a tile's
                                      // barriers never wait on other waves
-// tets a tile union may hold: the staged samples of 4 points per tet
-// (96 B) and the tet id (4 B) stay within 64 KB of LDS
-constexpr int TILE_MAX_UNION = 640;
+// tets a tile union may hold: two per lane (the staging and its prefetch
+// take a lane's two tets), 12.8 KB of LDS at 4 points per tet
+constexpr int TILE_MAX_UNION = 2 * TILE_B;
 
 struct FitArgs {
   LaunchShape shape;

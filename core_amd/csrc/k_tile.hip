@@ -171,77 +171,121 @@ __device__ __forceinline__ void sample_x(const MeshView& m, int32_t t, int l, do
   }
 }
 
-// Stages tets[0..U) of a tile: ids into sTet, the q sample points of each
-// into sX[(u q + l) 3 ..].  A lane per tet; the straight-sided path issues
-// the vertex ids of two tets, then their eight vertices' coordinates, before
-// it uses any of them, so each dependent load waits once per pair.
-template <bool P2MAP>
-__device__ __forceinline__ void stage_tets(const MeshView& m, const int32_t* tets, int U, int q,
-                                           int32_t* sTet, double* sX)
+// What a lane needs to start a tile, loaded while the previous tile is
+// fitted: the tile's tet range and lane range, the lane's two union tets and
+// their vertex ids (a lane stages tets lane and lane + 64), and the lane's
+// slot record.
+struct TileNext {
+  int32_t u0 = 0, U = 0, s0 = 0, nslot = 0;
+  int32_t ta = 0, tb = 0;
+  int4 va = {0, 0, 0, 0}, vb = {0, 0, 0, 0};
+  int4 sl = {0, 0, 0, 0};
+};
+
+// The prefetch in three steps, each placed after enough work for the loads
+// of the previous step to have arrived.
+__device__ __forceinline__ void next_meta(const TileView& tv, int64_t tile, TileNext& n)
 {
-  for (int u = threadIdx.x; u < U; u += 2 * TILE_B) {
-    const int ub = u + TILE_B;
-    const bool two = ub < U;
-    const int32_t ta = tets[u], tb = two ? tets[ub] : ta;
-    sTet[u] = ta;
-    if (two) sTet[ub] = tb;
+  if (tile < tv.ntile) {
+    n.u0 = tv.tet_off[tile];
+    n.U = tv.tet_off[tile + 1] - n.u0;
+    n.s0 = tv.slot_off[tile];
+    n.nslot = tv.slot_off[tile + 1] - n.s0;
+  } else {
+    n.U = 0;
+    n.nslot = 0;
+  }
+}
+
+__device__ __forceinline__ void next_ids(const TileView& tv, TileNext& n)
+{
+  const int l = threadIdx.x;
+  n.ta = l < n.U ? tv.tets[n.u0 + l] : 0;
+  n.tb = l + TILE_B < n.U ? tv.tets[n.u0 + l + TILE_B] : 0;
+  if (l < n.nslot) n.sl = tv.slot[n.s0 + l];
+}
+
+__device__ __forceinline__ void next_verts(const MeshView& m, TileNext& n)
+{
+  const int l = threadIdx.x;
+  if (l < n.U) n.va = *reinterpret_cast<const int4*>(m.tv + 4 * (int64_t)n.ta);
+  if (l + TILE_B < n.U) n.vb = *reinterpret_cast<const int4*>(m.tv + 4 * (int64_t)n.tb);
+}
+
+// Stages the tile's union into LDS: tet ids into sTet, the q sample points
+// of tet u into sX[(u q + l) 3 ..].  A lane per tet (at most 2 TILE_B tets
+// per tile: the union cap is below that for TILE_B = 64 lanes is checked by
+// the builder).  Straight-sided meshes use the prefetched vertex ids; curved
+// Lagrange-2 meshes map through all ten nodes.
+template <bool P2MAP>
+__device__ __forceinline__ void stage_tets(const MeshView& m, const TileNext& n, int q, int32_t* sTet,
+                                           double* sX)
+{
+  const int l = threadIdx.x;
+  for (int k = 0; k < 2; ++k) {
+    const int u = l + k * TILE_B;
+    if (u >= n.U) break;
+    const int32_t t = k == 0 ? n.ta : n.tb;
+    sTet[u] = t;
     if constexpr (P2MAP) {
-      for (int l = 0; l < q; ++l) {
+      for (int r = 0; r < q; ++r) {
         double x[3];
-        sample_x<true>(m, ta, l, x);
-        for (int j = 0; j < 3; ++j) sX[3 * (u * q + l) + j] = x[j];
-        if (two) {
-          sample_x<true>(m, tb, l, x);
-          for (int j = 0; j < 3; ++j) sX[3 * (ub * q + l) + j] = x[j];
-        }
+        sample_x<true>(m, t, r, x);
+        for (int j = 0; j < 3; ++j) sX[3 * (u * q + r) + j] = x[j];
       }
     } else {
-      const int4 va = *reinterpret_cast<const int4*>(m.tv + 4 * (int64_t)ta);
-      const int4 vb = *reinterpret_cast<const int4*>(m.tv + 4 * (int64_t)tb);
-      double P[2][4][3];
-      const int32_t ids[2][4] = {{va.x, va.y, va.z, va.w}, {vb.x, vb.y, vb.z, vb.w}};
+      const int4 v = k == 0 ? n.va : n.vb;
+      const int32_t ids[4] = {v.x, v.y, v.z, v.w};
+      double P[4][3];
 #pragma unroll
-      for (int k = 0; k < 2; ++k)
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) P[k][a][j] = m.xyz[3 * (int64_t)ids[k][a] + j];
-      for (int l = 0; l < q; ++l) {
+        for (int j = 0; j < 3; ++j) P[a][j] = m.xyz[3 * (int64_t)ids[a] + j];
+      for (int r = 0; r < q; ++r) {
         double xi[3];
-        rule_point(m.order, l, xi);
+        rule_point(m.order, r, xi);
         const double l0 = 1 - xi[0] - xi[1] - xi[2];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          if (k == 1 && !two) break;
-          const int uu = k == 0 ? u : ub;
-#pragma unroll
-          for (int j = 0; j < 3; ++j)
-            sX[3 * (uu * q + l) + j] =
-                fma(xi[2], P[k][3][j], fma(xi[1], P[k][2][j], fma(xi[0], P[k][1][j], l0 * P[k][0][j])));
-        }
+        for (int j = 0; j < 3; ++j)
+          sX[3 * (u * q + r) + j] = fma(xi[2], P[3][j], fma(xi[1], P[2][j], fma(xi[0], P[1][j], l0 * P[0][j])));
       }
     }
   }
 }
 
-// Lane l's value of x from lane l ^ st (st = 1, 2, 4), within rows of 16:
+// Lane l's value of x from lane l ^ ST (ST = 1, 2, 4), within rows of 16:
 // DPP quad permutations for 1 and 2, a swizzle for 4.
-__device__ __forceinline__ int xor_lane(int x, int st)
+template <int ST>
+__device__ __forceinline__ int xor_lane(int x)
 {
-  if (st == 1) return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
-  if (st == 2) return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
-  return __builtin_amdgcn_ds_swizzle(x, 0x101F);                                // xor 4 in 32 lanes
+  static_assert(ST == 1 || ST == 2 || ST == 4, "lane exchange distance");
+  if constexpr (ST == 1) return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+  if constexpr (ST == 2) return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+  return __builtin_amdgcn_ds_swizzle(x, 0x101F);                                          // xor 4 in 32 lanes
 }
 
-__device__ __forceinline__ double xor_lane(double x, int st)
+template <int ST>
+__device__ __forceinline__ double xor_lane(double x)
 {
-  const int lo = xor_lane(__double2loint(x), st), hi = xor_lane(__double2hiint(x), st);
+  const int lo = xor_lane<ST>(__double2loint(x)), hi = xor_lane<ST>(__double2hiint(x));
   return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ float xor_lane(float x, int st)
+template <int ST>
+__device__ __forceinline__ float xor_lane(float x)
 {
-  return __int_as_float(xor_lane(__float_as_int(x), st));
+  return __int_as_float(xor_lane<ST>(__float_as_int(x)));
+}
+
+template <int ST, class T, int N>
+__device__ __forceinline__ void group_step(T* v, int g)
+{
+  const bool in = g > ST;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const T t = xor_lane<ST>(v[k]);
+    v[k] = in ? v[k] + t : v[k];
+  }
 }
 
 // Sums v over the lanes of each group (g lanes, aligned); every lane of a
@@ -251,17 +295,9 @@ __device__ __forceinline__ float xor_lane(float x, int st)
 template <class T, int N>
 __device__ __forceinline__ void group_sum(T* v, int g, int gmax)
 {
-  for (int st = 1; st < gmax; st <<= 1) {
-    const bool in = g > st;
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      const T t = xor_lane(v[k], st);
-      v[k] = in ? v[k] + t : v[k];
-      // one exchange in flight at a time: otherwise the scheduler hoists all
-      // N of them and holds N temporaries next to the N sums
-      asm volatile("" : "+v"(v[k]));
-    }
-  }
+  if (gmax > 1) group_step<1, T, N>(v, g);
+  if (gmax > 2) group_step<2, T, N>(v, g);
+  if (gmax > 4) group_step<4, T, N>(v, g);
 }
 
 // Forward and back substitution with the Cholesky factor whose diagonal
@@ -294,34 +330,29 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
   const MeshView& m = a.m;
   const TileView& tv = a.tiles;
   const int q = a.q;
-  double* const sX = smem;                                           // [umax*q][3]
+  double* const sX = smem;                                                         // [umax*q][3]
   int32_t* const sTet = reinterpret_cast<int32_t*>(smem + 3 * tv.max_union * q);  // [umax]
   constexpr int NG = NT * (NT + 1) / 2;
-  for (int64_t tile = xcd_block(blockIdx.x, gridDim.x, XCD_K_TILE); tile < tv.ntile; tile += gridDim.x) {
-    // stage the union's tet ids and sample points, a lane per tet
-    const int32_t u0 = tv.tet_off[tile], U = tv.tet_off[tile + 1] - u0;
-    stage_tets<P2MAP>(m, tv.tets + u0, U, q, sTet, sX);
+  int64_t tile = xcd_block(blockIdx.x, gridDim.x, XCD_K_TILE);
+  TileNext nx;
+  next_meta(tv, tile, nx);
+  next_ids(tv, nx);
+  next_verts(m, nx);
+  for (; tile < tv.ntile; tile += gridDim.x) {
+    const TileNext cur = nx;
+    stage_tets<P2MAP>(m, cur, q, sTet, sX);
+    next_meta(tv, tile + gridDim.x, nx);  // arrives during the Gram sums
     __syncthreads();
-    // this lane's slot: patch, group size, rank in the group
-    const int32_t s0 = tv.slot_off[tile];
-    const bool has = threadIdx.x < tv.slot_off[tile + 1] - s0;
-    const uint32_t sv = has ? (uint32_t)tv.slot[s0 + threadIdx.x] : 0u;
+    // this lane's slot: patch, group size, rank in the group, set
+    const bool has = (int)threadIdx.x < cur.nslot;
+    const uint32_t sv = has ? (uint32_t)cur.sl.x : 0u;
     const int64_t p = sv & 0x7ffffffu;
     const int g = has ? 1 << ((sv >> 27) & 3u) : 1;
     const int rank = (int)(sv >> 29);
+    const uint16_t* const loc = tv.loc + (has ? cur.sl.y : 0);
+    const int ns = has ? cur.sl.z : 0;
     const uint64_t b2 = __ballot(g >= 2), b4 = __ballot(g >= 4), b8 = __ballot(g >= 8);
     const int gmax = b8 ? 8 : b4 ? 4 : b2 ? 2 : 1;
-    const uint16_t* loc = nullptr;
-    int ns = 0;
-    if (has) {
-      if (p < m.nv) {
-        loc = tv.v2t_loc + m.v2t_off[p];
-        ns = m.v2t_off[p + 1] - m.v2t_off[p];
-      } else {
-        loc = tv.e2t_loc + m.e2t_off[p - m.nv];
-        ns = m.e2t_off[p - m.nv + 1] - m.e2t_off[p - m.nv];
-      }
-    }
     // fewer samples than terms (hasEnoughPoints): straight to expansion
     const bool short_set = has && ns * q < NT;
     const bool work = has && !short_set;
@@ -348,6 +379,7 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
             for (int j = 0; j <= i; ++j) G[tri(i, j)] = fma(f[i], f[j], G[tri(i, j)]);
         }
       }
+    next_ids(tv, nx);  // arrives during the reduction and the factorisation
     group_sum<T, NG>(G, g, gmax);
     // Cholesky in place, with the rank certificate and the conditioning
     bool rank_ok = true;
@@ -446,6 +478,7 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
     wave_append(ACT_FALLBACK, app, a.L.count + 2 * NBUCKET + 1, a.L.fbq, (int32_t)p);
     for (int b = 0; b < NBUCKET; ++b)
       wave_append(b, app == ACT_REFIT ? bucket : -1, a.L.count + b, a.L.items[b], (int32_t)p);
+    next_verts(m, nx);  // arrives during the weighted sum
     // weighted sum over the lane's tets, centred on the set's first sample
     const bool solve = act == ACT_SOLVE;
     double v[NC], b0[NC];
@@ -465,20 +498,23 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
 #pragma unroll
         for (int c = 0; c < NC; ++c) v[c] = fma(w, b[c] - b0[c], v[c]);
       };
-      for (int s = rank; s < ns; s += g) {
-        const int u = loc[s];
-        const double* X = sX + 3 * q * u;
-        const double* bt = a.ip + (int64_t)sTet[u] * q * NC;
-        if (q == 4 && 4 * NC <= 36) {
-          // the tet's whole q x nc block in flight at once (G is dead here)
-          double b[4][NC];
+      if (q == 4 && 4 * NC <= 36) {
+        // the tet's whole q x nc block in flight at once (G is dead here)
+        for (int s = rank; s < ns; s += g) {
+          const int u = loc[s];
+          const double* X = sX + 12 * u;
+          const double* bt = a.ip + (int64_t)sTet[u] * 4 * NC;
+          double b[4 * NC];
 #pragma unroll
-          for (int l = 0; l < 4; ++l)
+          for (int i = 0; i < 4 * NC; ++i) b[i] = bt[i];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) b[l][c] = bt[l * NC + c];
-#pragma unroll
-          for (int l = 0; l < 4; ++l) accumulate(X + 3 * l, b[l]);
-        } else {
+          for (int l = 0; l < 4; ++l) accumulate(X + 3 * l, b + l * NC);
+        }
+      } else {
+        for (int s = rank; s < ns; s += g) {
+          const int u = loc[s];
+          const double* X = sX + 3 * q * u;
+          const double* bt = a.ip + (int64_t)sTet[u] * q * NC;
           for (int l = 0; l < q; ++l) {
             double b[NC];
 #pragma unroll

@@ -61,7 +61,7 @@ struct sprg_mesh {
   int straight = 1;
   DBuf tv, te, xyz, v2t_off, v2t, e2t_off, e2t;
   // tiles of the fast forms' regular pass (k_tile.hip)
-  DBuf t_slot_off, t_slot, t_tet_off, t_tets, t_v2t_loc, t_e2t_loc, t_leftover;
+  DBuf t_slot_off, t_slot, t_tet_off, t_tets, t_loc, t_leftover;
   int64_t ntile = 0, nleftover = 0;
   int max_union = 0;
   // per-call workspace
@@ -102,11 +102,10 @@ struct sprg_mesh {
     t.ntile = ntile;
     t.max_union = max_union;
     t.slot_off = t_slot_off.as<int32_t>();
-    t.slot = t_slot.as<int32_t>();
+    t.slot = t_slot.as<int4>();
     t.tet_off = t_tet_off.as<int32_t>();
     t.tets = t_tets.as<int32_t>();
-    t.v2t_loc = t_v2t_loc.as<uint16_t>();
-    t.e2t_loc = t_e2t_loc.as<uint16_t>();
+    t.loc = t_loc.as<uint16_t>();
     t.leftover = t_leftover.as<int32_t>();
     t.nleftover = nleftover;
     return t;
@@ -220,6 +219,7 @@ int build_tiles(sprg_mesh* m, const int32_t* tet_verts, const int32_t* tet_edges
   const bool edges = m->order == 2;
   const int64_t ne = edges ? m->ne : 0;
   if (nv + ne >= (1 << 27)) return 0;  // the slot encoding holds 27-bit patch ids
+  if (10 * nt >= INT32_MAX) return 0;  // loc offsets are int32
   std::vector<int32_t> vcount(nv, 0), ecount(ne, 0), owner(ne, -1);
   for (int64_t i = 0; i < 4 * nt; ++i) ++vcount[tet_verts[i]];
   if (edges)
@@ -306,8 +306,14 @@ int build_tiles(sprg_mesh* m, const int32_t* tet_verts, const int32_t* tet_edges
   m->nleftover = 0;
   if (!ntile) return 0;
   for (int64_t i = 0; i < ntile; ++i) counts[i + 1] += counts[i];
-  // slots: every tile's patches in descending group size, groups aligned
-  std::vector<int32_t> slot_off(ntile + 1, 0), slot;
+  // slots: every tile's patches in descending group size, groups aligned;
+  // each with the start and size of its set in the loc array (the CSR
+  // offsets are the prefix sums of the valences)
+  std::vector<int64_t> vfirst(nv + 1, 0), efirst(ne + 1, 0);
+  for (int64_t v = 0; v < nv; ++v) vfirst[v + 1] = vfirst[v] + vcount[v];
+  for (int64_t e = 0; e < ne; ++e) efirst[e + 1] = efirst[e] + ecount[e];
+  std::vector<int32_t> slot_off(ntile + 1, 0);
+  std::vector<int4> slot;
   slot.reserve((size_t)(4 * nv + ne));
   std::vector<std::pair<int, int32_t>> pat;
   for (int64_t k = 0; k < ntile; ++k) {
@@ -319,26 +325,36 @@ int build_tiles(sprg_mesh* m, const int32_t* tet_verts, const int32_t* tet_edges
     }
     std::stable_sort(pat.begin(), pat.end(),
                      [](const std::pair<int, int32_t>& x, const std::pair<int, int32_t>& y) { return x.first > y.first; });
-    for (const auto& pr : pat)
-      for (int r = 0; r < (1 << pr.first); ++r)
-        slot.push_back((int32_t)((uint32_t)pr.second | ((uint32_t)pr.first << 27) | ((uint32_t)r << 29)));
+    for (const auto& pr : pat) {
+      const int32_t p = pr.second;
+      const int64_t first = p < nv ? vfirst[p] : 4 * nt + efirst[p - nv];
+      const int32_t ns = p < nv ? vcount[p] : ecount[p - nv];
+      for (int r = 0; r < (1 << pr.first); ++r) {
+        int4 sl;
+        sl.x = (int32_t)((uint32_t)p | ((uint32_t)pr.first << 27) | ((uint32_t)r << 29));
+        sl.y = (int32_t)first;
+        sl.z = ns;
+        sl.w = 0;
+        slot.push_back(sl);
+      }
+    }
     slot_off[k + 1] = (int32_t)slot.size();
   }
-  if ((rc = m->t_slot_off.ensure(sizeof(int32_t) * (ntile + 1))) || (rc = m->t_slot.ensure(sizeof(int32_t) * slot.size())) ||
+  if ((rc = m->t_slot_off.ensure(sizeof(int32_t) * (ntile + 1))) || (rc = m->t_slot.ensure(sizeof(int4) * slot.size())) ||
       (rc = m->t_tets.ensure(sizeof(int32_t) * ((size_t)counts[ntile] + 1))) ||
-      (rc = m->t_v2t_loc.ensure(sizeof(uint16_t) * 4 * nt)) ||
-      (rc = m->t_e2t_loc.ensure(sizeof(uint16_t) * (edges ? 6 * nt : 1))) ||
+      (rc = m->t_loc.ensure(sizeof(uint16_t) * (edges ? 10 * nt : 4 * nt))) ||
       (rc = m->t_leftover.ensure(sizeof(int32_t) * (leftover.size() + 1))))
     return rc;
   HIP_OK(hipMemcpyAsync(m->t_tet_off.p, counts.data(), sizeof(int32_t) * (ntile + 1), hipMemcpyHostToDevice, s), "tiles");
   HIP_OK(hipMemcpyAsync(m->t_slot_off.p, slot_off.data(), sizeof(int32_t) * (ntile + 1), hipMemcpyHostToDevice, s), "tiles");
-  HIP_OK(hipMemcpyAsync(m->t_slot.p, slot.data(), sizeof(int32_t) * slot.size(), hipMemcpyHostToDevice, s), "tiles");
+  HIP_OK(hipMemcpyAsync(m->t_slot.p, slot.data(), sizeof(int4) * slot.size(), hipMemcpyHostToDevice, s), "tiles");
   if (!leftover.empty())
     HIP_OK(hipMemcpyAsync(m->t_leftover.p, leftover.data(), sizeof(int32_t) * leftover.size(), hipMemcpyHostToDevice, s),
            "tiles");
   if ((rc = launch_tile_build(mv, ntile, d_vert.as<int32_t>(), edges ? d_own_off.as<int32_t>() : nullptr,
                               d_own_e.as<int32_t>(), 1, m->t_tet_off.as<int32_t>(), m->t_tets.as<int32_t>(),
-                              m->t_v2t_loc.as<uint16_t>(), m->t_e2t_loc.as<uint16_t>(), d_status.as<int32_t>(), s)))
+                              m->t_loc.as<uint16_t>(), m->t_loc.as<uint16_t>() + 4 * nt, d_status.as<int32_t>(),
+                              s)))
     return rc;
   int32_t st = 0;
   HIP_OK(hipMemcpyAsync(&st, d_status.p, sizeof st, hipMemcpyDeviceToHost, s), "tiles");
@@ -443,7 +459,7 @@ extern "C" int sprg_mesh_destroy(sprg_mesh* m)
   DBuf* all[] = {&m->tv, &m->te, &m->xyz, &m->v2t_off, &m->v2t, &m->e2t_off, &m->e2t,
                  &m->lists, &m->xset, &m->rec, &m->elem, &m->partial, &m->scalars,
                  &m->status, &m->setlen, &m->t_slot_off, &m->t_slot, &m->t_tet_off,
-                 &m->t_tets, &m->t_v2t_loc, &m->t_e2t_loc, &m->t_leftover};
+                 &m->t_tets, &m->t_loc, &m->t_leftover};
   for (DBuf* b : all) b->release();
   for (int i = 0; i <= NPHASE; ++i)
     if (m->ev[i]) (void)hipEventDestroy(m->ev[i]);
