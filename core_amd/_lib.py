@@ -50,6 +50,7 @@ SIGNATURES = {
     "sprg_grad_ip": (_c_int, [_vp, _c_int, _c_int, _vp, _c_int, _vp]),
     "sprg_patch_sets": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp]),
     "sprg_recover": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int]),
+    "sprg_recover_noalloc": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int]),
     "sprg_spr_size": (_c_int, [_vp, _c_int, _c_int, _vp, _c_dbl, _vp, _c_int, _vp]),
     "sprg_target_spr_size": (_c_int, [_vp, _c_int, _c_int, _vp, ctypes.c_uint64, _c_dbl, _c_dbl,
                                       _vp, _c_int, _vp]),

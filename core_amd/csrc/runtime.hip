@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -21,6 +22,7 @@ constexpr int NPHASE = 5;
 struct sprg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  void* scratch = nullptr;  // this device's address of `scratch` (below)
 };
 
 namespace {
@@ -119,6 +121,39 @@ struct sprg_mesh {
       return sprg_fail(SPRG_EHIP, "%s: %s", what, hipGetErrorString(e_));     \
   } while (0)
 
+// Scratch of the allocation-free entry point (sprg_recover_noalloc): the
+// work lists and expanded-set storage of one recovery.  A file-scope device
+// array, so that entry point never allocates; every GPU the library runs on
+// holds its own copy (SPRG_NOALLOC_SCRATCH_BYTES) until the process ends.
+// Host code uses only the per-device address from hipGetSymbolAddress, never
+// the array's name.
+static __device__ __align__(256) char scratch[SPRG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once
+// per device.
+int scratch_address(int device, void** out)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= MAX_DEVICES) return sprg_fail(SPRG_EUNSUPPORTED, "device %d: at most %d devices", device, MAX_DEVICES);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return sprg_fail(SPRG_EHIP, "scratch address on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+
+}  // namespace
+
 extern "C" const char* sprg_version(void) { return "sprgpu 0.1 gfx950"; }
 
 extern "C" int sprg_ctx_create(int device, sprg_ctx** out)
@@ -136,6 +171,11 @@ extern "C" int sprg_ctx_create(int device, sprg_ctx** out)
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return sprg_fail(SPRG_EHIP, "ctx_create: stream creation failed");
+  }
+  if (int rc = scratch_address(device, &c->scratch)) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
   }
   *out = c;
   return 0;
@@ -536,15 +576,46 @@ void collect_phases(sprg_mesh* m)
 }
 
 // Lays out the work lists in one device allocation.
+// Bytes of the work lists of np patches: items and xitems per bucket, xq,
+// xq_exact, fbq, x_patch, x_len (int32 each), x_off (int64), counters,
+// cursors and the status words.
+size_t list_bytes(int64_t np)
+{
+  const size_t n32 = (size_t)np * (2 * NBUCKET + 5) + 2 * NBUCKET + 8;
+  return n32 * 4 + (size_t)np * 8 + 64;
+}
+
+// Lays the work lists of cap patches out from base (8-byte aligned); the
+// expanded sets live in x_set[0, x_cap).
+void carve_lists(char* base, int64_t cap, int32_t* x_set, int64_t x_cap, Lists& L)
+{
+  int64_t* x_off = (int64_t*)base;  // 8-byte aligned first
+  int64_t* x_cursor = x_off + cap;
+  int32_t* p = (int32_t*)(x_cursor + 2);
+  L.x_off = x_off;
+  L.x_cursor = x_cursor;
+  L.count = p;
+  p += 2 * NBUCKET + 2;
+  L.status = p;  // [0] status bits, [1] exact-expansion count, [2] fallback entries taken before the expansion
+  p += 4;
+  for (int b = 0; b < NBUCKET; ++b) { L.items[b] = p; p += cap; }
+  for (int b = 0; b < NBUCKET; ++b) { L.xitems[b] = p; p += cap; }
+  L.xq = p; p += cap;
+  L.xq_exact = p; p += cap;
+  L.fbq = p; p += cap;
+  L.x_patch = p; p += cap;
+  L.x_len = p; p += cap;
+  L.x_set = x_set;
+  L.x_cap = x_cap;
+  L.cap = cap;
+}
+
+// Lays the work lists out in the mesh's own workspaces (grown as needed).
 int setup_lists(sprg_mesh* m, FitArgs& a)
 {
   const int64_t np = m->npatch();
   if (m->list_cap < np) {
-    // items and xitems per bucket, xq, xq_exact, fbq, x_patch, x_len:
-    // int32; x_off: int64; counts, cursors, status
-    size_t n32 = (size_t)np * (2 * NBUCKET + 5) + 2 * NBUCKET + 8;
-    size_t bytes = n32 * 4 + (size_t)np * 8 + 64;
-    int rc = m->lists.ensure(bytes);
+    int rc = m->lists.ensure(list_bytes(np));
     if (rc) return rc;
     m->list_cap = np;
   }
@@ -554,27 +625,7 @@ int setup_lists(sprg_mesh* m, FitArgs& a)
     if (rc) return rc;
     m->xset_cap = cap;
   }
-  char* base = m->lists.as<char>();
-  int64_t* x_off = (int64_t*)base;  // 8-byte aligned first
-  int64_t* x_cursor = x_off + m->list_cap;
-  int32_t* p = (int32_t*)(x_cursor + 2);
-  Lists& L = a.L;
-  L.x_off = x_off;
-  L.x_cursor = x_cursor;
-  L.count = p;
-  p += 2 * NBUCKET + 2;
-  L.status = p;  // [0] status bits, [1] exact-expansion count, [2] fallback entries taken before the expansion
-  p += 4;
-  for (int b = 0; b < NBUCKET; ++b) { L.items[b] = p; p += m->list_cap; }
-  for (int b = 0; b < NBUCKET; ++b) { L.xitems[b] = p; p += m->list_cap; }
-  L.xq = p; p += m->list_cap;
-  L.xq_exact = p; p += m->list_cap;
-  L.fbq = p; p += m->list_cap;
-  L.x_patch = p; p += m->list_cap;
-  L.x_len = p; p += m->list_cap;
-  L.x_set = m->xset.as<int32_t>();
-  L.x_cap = m->xset_cap;
-  L.cap = m->list_cap;
+  carve_lists(m->lists.as<char>(), m->list_cap, m->xset.as<int32_t>(), m->xset_cap, a.L);
   return 0;
 }
 
@@ -711,6 +762,40 @@ int run_recover(sprg_mesh* m, int q, int nc, const double* ip, double* out, int 
   return sprg_fail(SPRG_ENOMEM, "SPR: expanded patch storage kept overflowing");
 }
 
+// The recovery with its work lists in the device scratch array: no
+// allocation, no growth.  The lists take list_bytes(npatch); the rest of the
+// scratch holds the expanded sets.
+int run_recover_noalloc(sprg_mesh* m, int q, int nc, const double* ip, double* out, int form)
+{
+  if (!valid_form(form)) return sprg_fail(SPRG_EINVAL, "unknown numeric form %d", form);
+  if (form != SPRG_FP64 && q > 4)
+    return sprg_fail(SPRG_EUNSUPPORTED, "the fast forms take at most 4 points per element");
+  char* base = static_cast<char*>(m->ctx->scratch);
+  if (!base) return sprg_fail(SPRG_EHIP, "recover_noalloc: no scratch on this context's device");
+  const int64_t np = m->npatch();
+  const size_t lists = (list_bytes(np) + 255) / 256 * 256;
+  // at least one expanded-set entry per patch plus slack beyond the lists
+  const size_t need = lists + sizeof(int32_t) * ((size_t)np + (1 << 18));
+  if (need > SPRG_NOALLOC_SCRATCH_BYTES)
+    return sprg_fail(SPRG_EINVAL,
+                     "recover_noalloc: %lld patches need %zu bytes of scratch, above the %zu this entry point "
+                     "holds (use sprg_recover)",
+                     (long long)np, need, (size_t)SPRG_NOALLOC_SCRATCH_BYTES);
+  FitArgs a = fit_args(m, q, nc, ip, out, form);
+  const int64_t x_cap = (int64_t)((SPRG_NOALLOC_SCRATCH_BYTES - lists) / sizeof(int32_t));
+  carve_lists(base, np, reinterpret_cast<int32_t*>(base + lists), x_cap, a.L);
+  hipStream_t s = m->ctx->stream;
+  int rc = enqueue_recover(m, a, form, s, true);
+  if (rc) return rc;
+  int32_t st = 0;
+  if ((rc = read_status(m, a, &st))) return rc;
+  collect_phases(m);
+  if (st & ST_XSET_OVERFLOW)
+    return sprg_fail(SPRG_ENOMEM, "recover_noalloc: the expanded patch sets overflowed the scratch (use sprg_recover)");
+  bool retry;
+  return status_code(m, st, &retry);
+}
+
 }  // namespace
 
 extern "C" int sprg_grad_ip(sprg_mesh* m, int field_order, int nc_in, const double* nodal,
@@ -740,6 +825,16 @@ extern "C" int sprg_recover(sprg_mesh* m, int ip_order, int nc, const double* ip
   if (rc) return rc;
   HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
   return run_recover(m, q, nc, ip, nodal, form);
+}
+
+extern "C" int sprg_recover_noalloc(sprg_mesh* m, int ip_order, int nc, const double* ip,
+                                    double* nodal, int form)
+{
+  if (!m || !ip || !nodal) return sprg_fail(SPRG_EINVAL, "recover_noalloc: null argument");
+  int q, rc = check_ip(m, ip_order, nc, &q);
+  if (rc) return rc;
+  HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  return run_recover_noalloc(m, q, nc, ip, nodal, form);
 }
 
 namespace {

@@ -146,6 +146,22 @@ int sprg_patch_sets(sprg_mesh* mesh, int ip_order, int64_t* set_off,
 int sprg_recover(sprg_mesh* mesh, int ip_order, int nc, const double* ip,
                  double* nodal, int form);
 
+/* ---- spr::recoverField without allocation ------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same recovery as sprg_recover, bit for bit, with
+ * its work lists and expanded patch sets in a device array of
+ * SPRG_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds from
+ * its first context to the end of the process; it allocates nothing.  The
+ * mesh (sprg_mesh_create) and the fields are the caller's.  Meshes whose
+ * lists do not fit (about 500,000 patches) get SPRG_EINVAL, expanded sets
+ * that overflow the rest SPRG_ENOMEM; use sprg_recover for those.  Every
+ * call on one GPU shares the array: calls that overlap on one GPU (two
+ * contexts' streams, or two host threads) race. */
+#define SPRG_NOALLOC_SCRATCH_BYTES (64u << 20)
+int sprg_recover_noalloc(sprg_mesh* mesh, int ip_order, int nc, const double* ip,
+                         double* nodal, int form);
+
 /* ---- spr::getSPRSizeField (spr/sprEstimateError.cc:311-321) ------------
  * ip [nt][q][nc] -> size [nv].  diag (host, optional, 4 doubles):
  * ||eps*||^2, sum of element error terms, size_factor, wall ms. */
