@@ -59,6 +59,9 @@ SIGNATURES = {
     "sprg_batch_destroy": (_c_int, [_vp]),
     "sprg_batch_size": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_dbl, _c_dbl, _c_dbl, _vp, _c_int,
                                  _vp]),
+    "sprg_smb_counts": (_c_int, [ctypes.c_char_p, _vp, _vp]),
+    "sprg_smb_read": (_c_int, [ctypes.c_char_p, _vp, _vp, _vp, _vp]),
+    "sprg_mesh_load_smb": (_c_int, [_vp, ctypes.c_char_p, _c_int, _pp]),
     "sprg_box_counts": (_c_int, [_c_int, _vp, _vp, _vp]),
     "sprg_box_build": (_c_int, [_c_int, _c_int, _c_dbl, ctypes.c_uint64, _c_int, _vp, _vp, _vp,
                                 _vp]),

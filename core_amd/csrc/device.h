@@ -26,8 +26,10 @@ struct MeshView {
 };
 
 // Patch work lists.  Bucket b holds patches whose row count m = q*|set|
-// fits rows_of_bucket(b).
+// fits rows_of_bucket(b); larger sets, up to BIG_MMAX rows, go to the big
+// list.
 enum { NBUCKET = 6 };
+constexpr int BIG_MMAX = 512;
 __host__ __device__ inline int bucket_rows(int b)
 {
   const int r[NBUCKET] = {16, 32, 64, 96, 128, 192};
@@ -43,7 +45,7 @@ __host__ __device__ inline int bucket_of(int m)
 // Status bits written by kernels (read back by the host after a call).
 enum {
   ST_PATCH_FAIL = 1,     // "all hope is lost"
-  ST_TOO_LARGE = 2,      // a patch exceeded the supported row count
+  ST_TOO_LARGE = 2,      // a set exceeded BIG_MMAX rows, or an expansion 192
   ST_XSET_OVERFLOW = 4,  // expanded-set storage too small
   ST_INTERNAL = 8,
   ST_ILLCOND = 16        // a patch too ill-conditioned for the FP64 fallback
@@ -60,6 +62,10 @@ struct Lists {
                              // follows the status word
   int32_t* fbq;              // fast forms: FP64 fallback queue; bit 31: an
                              // expanded item rather than a patch id
+  int32_t* big;              // sets of more than 192 (at most BIG_MMAX) rows
+                             // for the exact kernel k_exact_big; bit 31: an
+                             // expanded item, bit 30: rank not yet decided.
+                             // Its count is status[3].
   // expanded sets
   int32_t* x_patch;
   int64_t* x_off;
@@ -170,6 +176,10 @@ int launch_classify(const FitArgs& a, int nterms, hipStream_t s);
 // (items[] lists holding patch ids or, with bit 31, expanded-set ids).
 int launch_exact_fit(const FitArgs& a, int nterms, int mode, hipStream_t s);
 int launch_expand(const FitArgs& a, int nterms, hipStream_t s);
+// The exact fits of the big list (one wave per set, workspace ws of
+// big_ws_bytes(grid) bytes); grid 0 launches nothing.
+size_t big_ws_bytes(int grid);
+int launch_exact_big(const FitArgs& a, int nterms, double* ws, int grid, hipStream_t s);
 int launch_fast_fit(const FitArgs& a, int nterms, int form, int src, hipStream_t s);
 // The regular pass of the fast forms over the mesh's tiles (k_tile.hip).
 int launch_tile_fit(const FitArgs& a, int nterms, int form, hipStream_t s);

@@ -6,13 +6,23 @@
 
 #include <hip/hip_runtime.h>
 
+#include "device.h"
+
 namespace sprg {
 
 // What a fit pass decides for a patch.  ACT_EXACT: the exact path decides
 // the rank (and expands); ACT_FALLBACK: an FP64 pass refits it; ACT_REFIT:
 // the exact kernel refits the certified set with the reference's
-// arithmetic; ACT_ILLCOND: no form here can fit it.
-enum { ACT_SOLVE = 0, ACT_EXACT = 1, ACT_FALLBACK = 2, ACT_ILLCOND = 3, ACT_NONE = 4, ACT_REFIT = 5 };
+// arithmetic; ACT_BIG: the same for a set above 192 rows (k_exact_big);
+// ACT_ILLCOND: no form here can fit it.
+enum { ACT_SOLVE = 0, ACT_EXACT = 1, ACT_FALLBACK = 2, ACT_ILLCOND = 3, ACT_NONE = 4, ACT_REFIT = 5, ACT_BIG = 6 };
+
+// Appends v to the big list (count: status[3]); every lane of the wave
+// calls it.
+__device__ __forceinline__ void big_append(bool mine, const Lists& L, int32_t v)
+{
+  wave_append(1, mine ? 1 : 0, L.status + 3, L.big, v);
+}
 
 template <class S, int NT>
 __device__ __forceinline__ void basis(const double* d, S* f)

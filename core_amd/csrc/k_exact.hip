@@ -23,6 +23,7 @@
 //  - rows of the patch matrix are recomputed from the mesh: sample points
 //    are the mesh-order rule points mapped by the coordinate field.
 #include "device.h"
+#include "fit_common.h"
 #include "spr_math.h"
 
 namespace sprg {
@@ -147,7 +148,9 @@ __device__ __forceinline__ void patch_set(const FitArgs& a, int64_t p,
 __global__ __launch_bounds__(256) void k_classify(FitArgs a, int nterms)
 {
   int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  int target = -1;  // -1 none, 0..NBUCKET-1 bucket, NBUCKET expansion queue
+  // -1 none, 0..NBUCKET-1 bucket, NBUCKET expansion queue, NBUCKET + 1 the
+  // big list (the rank is decided there)
+  int target = -1;
   if (p < a.npatch) {
     const int32_t* set;
     int ns;
@@ -160,14 +163,15 @@ __global__ __launch_bounds__(256) void k_classify(FitArgs a, int nterms)
     } else {
       target = bucket_of(m);
       if (target >= NBUCKET) {
-        atomicOr(a.L.status, ST_TOO_LARGE);
-        target = -1;
+        target = m <= BIG_MMAX ? NBUCKET + 1 : -1;
+        if (target < 0) atomicOr(a.L.status, ST_TOO_LARGE);
       }
     }
   }
   for (int b = 0; b < NBUCKET; ++b)
     wave_append(b, target, a.L.count + b, a.L.items[b], (int32_t)p);
   wave_append(NBUCKET, target, a.L.count + 2 * NBUCKET, a.L.xq, (int32_t)p);
+  big_append(target == NBUCKET + 1, a.L, (int32_t)(p | 0x40000000));
 }
 
 // MODE: FIT_REGULAR walks items[bucket] (patch ids), FIT_EXPANDED walks
@@ -448,6 +452,19 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
       __syncthreads();
       continue;
     }
+    if (!EXACT && test_first && ns0 * q > XMAXR) {
+      // above the expansion's rows: the big kernel decides the rank of the
+      // initial set (and fits it); this item keeps no set
+      if (lane == 0) {
+        if (ns0 * q <= BIG_MMAX) {
+          a.L.big[atomicAdd(a.L.status + 3, 1)] = (int32_t)(p | 0x40000000);
+        } else {
+          atomicOr(a.L.status, ST_TOO_LARGE);
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     if (!EXACT && test_first && ns0 * q >= NT) {
       if (lane == 0) a.L.xq_exact[atomicAdd(nexact, 1)] = it;
       __syncthreads();
@@ -617,6 +634,105 @@ __global__ __launch_bounds__(256) void k_bucket_items(FitArgs a)
   }
 }
 
+// ------------------------------------------------------------------------
+// Sets of more than 192 rows (up to BIG_MMAX): one wave per set, the same
+// arithmetic as k_exact_fit (mth/mthQR.cc:19-80, 132-163).  R and, later,
+// the first NT columns of Q live in global workspace, the reflectors in LDS;
+// lane j forms Q's row j (rows j, j+64, ...) in its own column of the
+// workspace (entry l of the row at [l][lane], so that the wave's accesses
+// are contiguous), with the reference's loops from k to m-1.
+
+constexpr size_t BIG_WS_DOUBLES = (size_t)BIG_MMAX * 10 + (size_t)64 * BIG_MMAX;
+
+template <int NT>
+__global__ __launch_bounds__(64) void k_exact_big(FitArgs a, double* ws)
+{
+  __shared__ double sV[NT * BIG_MMAX];
+  __shared__ double sRt[NT * NT];
+  __shared__ double sY[27 * NT];
+  __shared__ double sDot[NT];
+  __shared__ double sNorm[2];
+  __shared__ int32_t sS[BIG_MMAX];
+  const int lane = threadIdx.x;
+  const MeshView& m = a.m;
+  const int q = a.q, nc = a.nc;
+  double* const R = ws + blockIdx.x * BIG_WS_DOUBLES;  // [m][NT], then Q's first NT columns
+  double* const Qw = R + (size_t)BIG_MMAX * 10;        // [BIG_MMAX][64]
+  const int n = a.L.status[3];
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const int32_t entry = a.L.big[k];
+    const bool expanded = entry < 0, test = (entry & 0x40000000) != 0;
+    const int32_t id = entry & 0x07ffffff;
+    int64_t p;
+    const int32_t* set;
+    int ns;
+    if (expanded) {
+      p = a.L.x_patch[id];
+      set = a.L.x_set + a.L.x_off[id];
+      ns = a.L.x_len[id];
+    } else {
+      p = id;
+      patch_set(a, p, &set, &ns);
+    }
+    const int mrows = ns * q;  // <= BIG_MMAX (checked where the entry was made)
+    for (int i = lane; i < ns; i += 64) sS[i] = set[i];
+    __syncthreads();
+    build_rows<NT, 64>(m, q, sS, mrows, R, lane);
+    __syncthreads();
+    const bool ok = r_recurrence<NT, 64, BIG_MMAX>(R, sV, sDot, sNorm, mrows, lane, true);
+    if (!ok) {
+      // a set this large that is not of full rank would have to grow
+      // beyond what the expansion holds; a certified one is inconsistent
+      if (lane == 0) atomicOr(a.L.status, test ? ST_TOO_LARGE : ST_INTERNAL);
+      __syncthreads();
+      continue;
+    }
+    for (int i = lane; i < NT * NT; i += 64) sRt[i] = R[i];
+    __syncthreads();
+    // Q rows: q_j = e_j, then reflect_rows for k = 0..NT-1 (mth/mthQR.cc:65-80)
+    for (int j = lane; j < mrows; j += 64) {
+      for (int l = 0; l < mrows; ++l) Qw[(size_t)l * 64 + lane] = (l == j) ? 1.0 : 0.0;
+      for (int r = 0; r < NT; ++r) {
+        const double* v = sV + r * BIG_MMAX;
+        double dot = 0;
+        for (int l = r; l < mrows; ++l) dot += Qw[(size_t)l * 64 + lane] * v[l];
+        const double t2 = 2 * dot;
+        for (int l = r; l < mrows; ++l) Qw[(size_t)l * 64 + lane] -= t2 * v[l];
+      }
+      for (int i = 0; i < NT; ++i) R[j * NT + i] = Qw[(size_t)i * 64 + lane];
+    }
+    __syncthreads();
+    // y = transpose(Q) b, sequential over rows (mth/mth_def.h:221-232)
+    for (int w = lane; w < NT * nc; w += 64) {
+      const int i = w % NT, c = w / NT;
+      double y = 0;
+      for (int r = 0; r < mrows; ++r) y += R[r * NT + i] * a.ip[((int64_t)sS[r / q] * q + r % q) * nc + c];
+      sY[c * NT + i] = y;
+    }
+    __syncthreads();
+    // backsubUT (mth/mthQR.cc:131-148) and evalPolynomial at the node
+    double node[3], t[10];
+    for (int j = 0; j < 3; ++j) node[j] = m.xyz[3 * p + j];
+    poly_terms(NT == 4 ? 1 : 2, node, t);
+    for (int c = lane; c < nc; c += 64) {
+      double x[NT];
+#pragma unroll
+      for (int ii = 0; ii < NT; ++ii) {
+        const int i = NT - ii - 1;
+        x[i] = sY[c * NT + i];
+#pragma unroll
+        for (int jj = i + 1; jj < NT; ++jj) x[i] -= sRt[i * NT + jj] * x[jj];
+        x[i] /= sRt[i * NT + i];
+      }
+      double v = 0;
+#pragma unroll
+      for (int i = 0; i < NT; ++i) v += x[i] * t[i];
+      a.out[p * nc + c] = v;
+    }
+    __syncthreads();
+  }
+}
+
 inline unsigned grid_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
 template <int NT, int EXP>
@@ -651,6 +767,18 @@ int launch_exact_fit(const FitArgs& a, int nterms, int mode, hipStream_t s)
     case FIT_EXPANDED: return nterms == 4 ? fit_all<4, FIT_EXPANDED>(a, s) : fit_all<10, FIT_EXPANDED>(a, s);
     default: return nterms == 4 ? fit_all<4, FIT_MIXED>(a, s) : fit_all<10, FIT_MIXED>(a, s);
   }
+}
+
+size_t big_ws_bytes(int grid) { return sizeof(double) * BIG_WS_DOUBLES * (size_t)grid; }
+
+int launch_exact_big(const FitArgs& a, int nterms, double* ws, int grid, hipStream_t s)
+{
+  if (grid <= 0) return 0;
+  if (nterms == 4)
+    k_exact_big<4><<<(unsigned)grid, 64, 0, s>>>(a, ws);
+  else
+    k_exact_big<10><<<(unsigned)grid, 64, 0, s>>>(a, ws);
+  return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "big exact fit launch failed");
 }
 
 int launch_expand(const FitArgs& a, int nterms, hipStream_t s)

@@ -243,8 +243,9 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
     const int bucket = bucket_of(ns * q);
     const int poor = (regular && sizeof(T) == 4) ? ACT_FALLBACK
                      : bucket < NBUCKET                     ? ACT_REFIT
+                     : ns * q <= BIG_MMAX                   ? ACT_BIG
                      : src != SRC_FALLBACK && src != SRC_PRE_FALLBACK ? ACT_FALLBACK
-                     // more rows than the exact kernel takes: the refined
+                     // more rows than the exact kernels take: the refined
                      // FP64 solve, unless hopeless
                      : min_ratio >= 1e-13f ? ACT_SOLVE : ACT_ILLCOND;
     int act = ACT_NONE;
@@ -308,6 +309,7 @@ __global__ __launch_bounds__(256) void k_fast(FitArgs a, int src, float tau_rank
     for (int b = 0; b < NBUCKET; ++b)
       wave_append(b, act == ACT_REFIT ? bucket : -1, a.L.count + b, a.L.items[b],
                   expanded ? (int32_t)(id | 0x80000000u) : (int32_t)p);
+    big_append(act == ACT_BIG, a.L, expanded ? (int32_t)(id | 0x80000000u) : (int32_t)p);
     if (act == ACT_ILLCOND) atomicOr(a.L.status, ST_ILLCOND);
     if (act != ACT_SOLVE) continue;
     // weighted sum, centred on the first sample

@@ -409,7 +409,10 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
     }
     // the regular pass's routes (k_fast.hip, SRC_REGULAR)
     const int bucket = bucket_of(ns * q);
-    const int poor = sizeof(T) == 4 ? ACT_FALLBACK : bucket < NBUCKET ? ACT_REFIT : ACT_FALLBACK;
+    const int poor = sizeof(T) == 4        ? ACT_FALLBACK
+                     : bucket < NBUCKET      ? ACT_REFIT
+                     : ns * q <= BIG_MMAX    ? ACT_BIG
+                                             : ACT_FALLBACK;
     int act = ACT_NONE;
     if (short_set)
       act = ACT_EXACT;
@@ -478,6 +481,7 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
     wave_append(ACT_FALLBACK, app, a.L.count + 2 * NBUCKET + 1, a.L.fbq, (int32_t)p);
     for (int b = 0; b < NBUCKET; ++b)
       wave_append(b, app == ACT_REFIT ? bucket : -1, a.L.count + b, a.L.items[b], (int32_t)p);
+    big_append(app == ACT_BIG, a.L, (int32_t)p);
     next_verts(m, nx);  // arrives during the weighted sum
     // weighted sum over the lane's tets, centred on the set's first sample
     const bool solve = act == ACT_SOLVE;

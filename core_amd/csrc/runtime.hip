@@ -67,13 +67,14 @@ struct sprg_mesh {
   int64_t ntile = 0, nleftover = 0;
   int max_union = 0;
   // per-call workspace
-  DBuf lists, xset, rec, elem, partial, scalars, status, setlen;
+  DBuf lists, xset, rec, elem, partial, scalars, status, setlen, big_ws;
+  int max_set = 0;    // tets in the largest initial patch set
   int64_t list_cap = 0, xset_cap = 0;
   // phase timing of the last call: events on the context stream
   hipEvent_t ev[NPHASE + 1] = {};
   bool ev_ok = false;
   double phase_ms[NPHASE] = {};
-  int64_t last_stats[6] = {};
+  int64_t last_stats[7] = {};
   LaunchShape shape;  // launch shapes of the patch kernels (sprg_mesh_autotune)
   int phase_last = 0;  // highest phase recorded by the last call
   void mark(int i)
@@ -258,8 +259,6 @@ int build_tiles(sprg_mesh* m, const int32_t* tet_verts, const int32_t* tet_edges
   const int64_t nv = m->nv, nt = m->nt;
   const bool edges = m->order == 2;
   const int64_t ne = edges ? m->ne : 0;
-  if (nv + ne >= (1 << 27)) return 0;  // the slot encoding holds 27-bit patch ids
-  if (10 * nt >= INT32_MAX) return 0;  // loc offsets are int32
   std::vector<int32_t> vcount(nv, 0), ecount(ne, 0), owner(ne, -1);
   for (int64_t i = 0; i < 4 * nt; ++i) ++vcount[tet_verts[i]];
   if (edges)
@@ -272,6 +271,11 @@ int build_tiles(sprg_mesh* m, const int32_t* tet_verts, const int32_t* tet_edges
           owner[e] = a < b ? a : b;
         }
       }
+  m->max_set = 0;
+  for (int32_t c : vcount) m->max_set = std::max(m->max_set, (int)c);
+  for (int32_t c : ecount) m->max_set = std::max(m->max_set, (int)c);
+  if (nv + ne >= (1 << 27)) return 0;  // the slot encoding holds 27-bit patch ids
+  if (10 * nt >= INT32_MAX) return 0;  // loc offsets are int32
   std::vector<int32_t> own_off(nv + 1, 0), own_e(ne);
   for (int64_t e = 0; e < ne; ++e) ++own_off[owner[e] + 1];
   for (int64_t v = 0; v < nv; ++v) own_off[v + 1] += own_off[v];
@@ -498,7 +502,7 @@ extern "C" int sprg_mesh_destroy(sprg_mesh* m)
   hipStreamSynchronize(m->ctx->stream);
   DBuf* all[] = {&m->tv, &m->te, &m->xyz, &m->v2t_off, &m->v2t, &m->e2t_off, &m->e2t,
                  &m->lists, &m->xset, &m->rec, &m->elem, &m->partial, &m->scalars,
-                 &m->status, &m->setlen, &m->t_slot_off, &m->t_slot, &m->t_tet_off,
+                 &m->status, &m->setlen, &m->big_ws, &m->t_slot_off, &m->t_slot, &m->t_tet_off,
                  &m->t_tets, &m->t_loc, &m->t_leftover};
   for (DBuf* b : all) b->release();
   for (int i = 0; i <= NPHASE; ++i)
@@ -527,8 +531,8 @@ extern "C" int sprg_mesh_phase_times(sprg_mesh* m, double* ms, int n)
 extern "C" int sprg_mesh_last_stats(sprg_mesh* m, int64_t* stats, int n)
 {
   if (!m || (!stats && n > 0)) return sprg_fail(SPRG_EINVAL, "mesh_last_stats: null argument");
-  for (int i = 0; i < n && i < 6; ++i) stats[i] = m->last_stats[i];
-  return 6;
+  for (int i = 0; i < n && i < 7; ++i) stats[i] = m->last_stats[i];
+  return 7;
 }
 
 extern "C" int sprg_mesh_adjacency(sprg_mesh* m, int dim, const int32_t** off, const int32_t** idx)
@@ -576,12 +580,16 @@ void collect_phases(sprg_mesh* m)
 }
 
 // Lays out the work lists in one device allocation.
+// Waves of the exact kernel for sets above 192 rows: none unless an initial
+// set of this mesh has that many rows at q points per tet.
+int big_grid(const sprg_mesh* m, int q) { return (int64_t)m->max_set * q > bucket_rows(NBUCKET - 1) ? 128 : 0; }
+
 // Bytes of the work lists of np patches: items and xitems per bucket, xq,
 // xq_exact, fbq, x_patch, x_len (int32 each), x_off (int64), counters,
 // cursors and the status words.
 size_t list_bytes(int64_t np)
 {
-  const size_t n32 = (size_t)np * (2 * NBUCKET + 5) + 2 * NBUCKET + 8;
+  const size_t n32 = (size_t)np * (2 * NBUCKET + 6) + 2 * NBUCKET + 8;
   return n32 * 4 + (size_t)np * 8 + 64;
 }
 
@@ -603,6 +611,7 @@ void carve_lists(char* base, int64_t cap, int32_t* x_set, int64_t x_cap, Lists& 
   L.xq = p; p += cap;
   L.xq_exact = p; p += cap;
   L.fbq = p; p += cap;
+  L.big = p; p += cap;
   L.x_patch = p; p += cap;
   L.x_len = p; p += cap;
   L.x_set = x_set;
@@ -626,6 +635,10 @@ int setup_lists(sprg_mesh* m, FitArgs& a)
     m->xset_cap = cap;
   }
   carve_lists(m->lists.as<char>(), m->list_cap, m->xset.as<int32_t>(), m->xset_cap, a.L);
+  if (big_grid(m, a.q) > 0) {
+    int rc = m->big_ws.ensure(big_ws_bytes(big_grid(m, a.q)));
+    if (rc) return rc;
+  }
   return 0;
 }
 
@@ -634,7 +647,7 @@ int read_status(sprg_mesh* m, const FitArgs& a, int32_t* st)
   hipStream_t s = m->ctx->stream;
   // counts [0, 2 NBUCKET + 2), the status word and the exact-expansion
   // count follow each other
-  int32_t c[2 * NBUCKET + 4];
+  int32_t c[2 * NBUCKET + 6];
   HIP_OK(hipMemcpyAsync(c, a.L.count, sizeof c, hipMemcpyDeviceToHost, s), "status");
   HIP_OK(hipStreamSynchronize(s), "recover sync");
   *st = c[2 * NBUCKET + 2];
@@ -646,6 +659,7 @@ int read_status(sprg_mesh* m, const FitArgs& a, int32_t* st)
   m->last_stats[3] = c[2 * NBUCKET + 1];
   m->last_stats[4] = *st;
   m->last_stats[5] = c[2 * NBUCKET + 3];
+  m->last_stats[6] = c[2 * NBUCKET + 5];
   return 0;
 }
 
@@ -663,7 +677,8 @@ int enqueue_recover(sprg_mesh* m, const FitArgs& a, int form, hipStream_t s, boo
   const int nterms = m->order == 1 ? 4 : 10;
   int rc;
   // counters, status, the exact-expansion count and the fallback start
-  if ((rc = launch_copy_i32(a.L.count, nullptr, 2 * NBUCKET + 5, s))) return rc;
+  // (the status words follow the counters: bits, exact count, fallback start, big count)
+  if ((rc = launch_copy_i32(a.L.count, nullptr, 2 * NBUCKET + 6, s))) return rc;
   if ((rc = launch_copy_i32(reinterpret_cast<int32_t*>(a.L.x_cursor), nullptr, 4, s))) return rc;
   if (marks) {
     m->phase_last = 0;
@@ -699,6 +714,8 @@ int enqueue_recover(sprg_mesh* m, const FitArgs& a, int form, hipStream_t s, boo
   } else {
     if ((rc = launch_exact_fit(a, nterms, 1, s))) return rc;
   }
+  // sets above 192 rows, from every pass above
+  if ((rc = launch_exact_big(a, nterms, m->big_ws.as<double>(), big_grid(m, a.q), s))) return rc;
   if (marks) m->mark(3);
   return 0;
 }
@@ -733,7 +750,9 @@ int status_code(sprg_mesh* m, int32_t st, bool* retry)
   }
   if (st & ST_PATCH_FAIL) return sprg_fail(SPRG_EPATCH, "SPR: patch construction: all hope is lost.");
   if (st & ST_TOO_LARGE)
-    return sprg_fail(SPRG_EUNSUPPORTED, "SPR: a patch needs more than %d sample rows", bucket_rows(NBUCKET - 1));
+    return sprg_fail(SPRG_EUNSUPPORTED,
+                     "SPR: a patch set needs more than %d sample rows, or a set that must grow has more than %d",
+                     BIG_MMAX, bucket_rows(NBUCKET - 1));
   if (st & ST_INTERNAL) return sprg_fail(SPRG_EINTERNAL, "SPR: inconsistent rank decision");
   if (st & ST_ILLCOND)
     return sprg_fail(SPRG_EUNSUPPORTED,
@@ -781,6 +800,10 @@ int run_recover_noalloc(sprg_mesh* m, int q, int nc, const double* ip, double* o
                      "recover_noalloc: %lld patches need %zu bytes of scratch, above the %zu this entry point "
                      "holds (use sprg_recover)",
                      (long long)np, need, (size_t)SPRG_NOALLOC_SCRATCH_BYTES);
+  if (big_grid(m, q) > 0)
+    return sprg_fail(SPRG_EUNSUPPORTED,
+                     "recover_noalloc: patch sets above %d rows need the workspace of sprg_recover",
+                     bucket_rows(NBUCKET - 1));
   FitArgs a = fit_args(m, q, nc, ip, out, form);
   const int64_t x_cap = (int64_t)((SPRG_NOALLOC_SCRATCH_BYTES - lists) / sizeof(int32_t));
   carve_lists(base, np, reinterpret_cast<int32_t*>(base + lists), x_cap, a.L);
@@ -1247,7 +1270,7 @@ int workspace_key(sprg_batch* b, int q, int nc, int form, std::vector<uint64_t>&
     FitArgs a = fit_args(m, q, nc, nullptr, nullptr, form);
     if (!rc) rc = setup_lists(m, a);
     if (rc) return rc;
-    const DBuf* bufs[] = {&m->rec, &m->lists, &m->xset, &m->elem, &m->partial, &m->scalars};
+    const DBuf* bufs[] = {&m->rec, &m->lists, &m->xset, &m->elem, &m->partial, &m->scalars, &m->big_ws};
     for (const DBuf* d : bufs) {
       key.push_back((uint64_t)(uintptr_t)d->p);
       key.push_back((uint64_t)d->bytes);

@@ -74,6 +74,36 @@ def tet_rule(order):
     raise ValueError("integration order %d is not supported" % order)
 
 
+SMB_TYPES = ("vertex", "edge", "triangle", "quad", "hex", "prism", "pyramid", "tet")
+
+
+def smb_counts(path):
+    """Entity counts of an MDS .smb part file by SMB type, and its version."""
+    counts = np.zeros(8, np.int64)
+    version = np.zeros(1, np.int32)
+    _lib.check(_lib.lib().sprg_smb_counts(str(path).encode(), _ptr(counts), _ptr(version)))
+    return dict(zip(SMB_TYPES, counts.tolist())), int(version[0])
+
+
+def load_smb(path, order=1):
+    """One part of tetrahedra from an MDS .smb file (apf::loadMdsPart) in
+    the file's entity order.  order 2: Lagrange-2 coordinates with edge
+    nodes at the edge midpoints (apf::changeShape's projection)."""
+    counts, _ = smb_counts(path)
+    nv, ne, nt = counts["vertex"], counts["edge"], counts["tet"]
+    tv = np.empty((nt, 4), np.int32)
+    te = np.empty((nt, 6), np.int32)
+    ev = np.empty((ne, 2), np.int32)
+    xyz = np.empty((nv, 3), np.float64)
+    _lib.check(_lib.lib().sprg_smb_read(str(path).encode(), _ptr(tv), _ptr(te), _ptr(ev), _ptr(xyz)))
+    if order == 2:
+        mid = np.zeros((ne, 3))
+        mid += xyz[ev[:, 0]] * 0.5
+        mid += xyz[ev[:, 1]] * 0.5
+        xyz = np.vstack([xyz, mid])
+    return HostMesh(tv, xyz, order, te=te, ev=ev, ne=ne)
+
+
 def box_counts(n):
     nv = np.zeros(1, np.int64)
     nt = np.zeros(1, np.int64)

@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from ._lib import SPRError, SPRG_FP32R, SPRG_FP64, SPRG_FP64F
-from .mesh import HostMesh
+from .mesh import HostMesh, load_smb
 
 SCALAR, VECTOR, MATRIX = 1, 3, 9
 FORMS = {"fp64": SPRG_FP64, "fp32r": SPRG_FP32R, "fp64f": SPRG_FP64F}
@@ -76,7 +76,7 @@ class Field:
 class Mesh:
     """Device mirror of one tet mesh part (sprg_mesh_create)."""
 
-    def __init__(self, host, device=0):
+    def __init__(self, host, device=0, _handle=None):
         if not isinstance(host, HostMesh):
             raise TypeError("Mesh expects a HostMesh")
         self.host = host
@@ -86,6 +86,9 @@ class Mesh:
         self.nv, self.nt, self.ne = host.nv, host.nt, host.ne
         self.nnodes = host.nnodes
         self.fields = {}
+        if _handle is not None:
+            self.handle = _handle
+            return
         h = _lib.ctypes.c_void_p()
         L = _lib.lib()
         _lib.check(L.sprg_mesh_create(self.ctx, host.nv, host.nt, host.ne if host.te is not None else 0,
@@ -93,6 +96,16 @@ class Mesh:
                                       None if host.te is None else host.te.ctypes.data,
                                       host.xyz.ctypes.data, host.order, _lib.ctypes.byref(h)))
         self.handle = h
+
+    @classmethod
+    def load_smb(cls, path, order=1, device=0):
+        """apf::loadMdsPart + (order 2) changeShape to Lagrange-2: the device
+        mirror read by sprg_mesh_load_smb; .host holds the same arrays."""
+        host = load_smb(path, order)
+        h = _lib.ctypes.c_void_p()
+        _lib.check(_lib.lib().sprg_mesh_load_smb(context(device), str(path).encode(), order,
+                                                 _lib.ctypes.byref(h)))
+        return cls(host, device, _handle=h)
 
     def close(self):
         if self.handle:
@@ -167,10 +180,10 @@ class Mesh:
         """Patch routes of the last recovery: patches, queued for the exact
         rank test / expansion, expanded sets, FP64 fallback refits, status,
         queued patches whose expansion needed the reference's rank test."""
-        st = np.zeros(6, np.int64)
-        _lib.lib().sprg_mesh_last_stats(self.handle, st.ctypes.data, 6)
-        return dict(zip(["patches", "queued_exact", "expanded", "fallback", "status", "exact_tested"],
-                        st.tolist()))
+        st = np.zeros(7, np.int64)
+        _lib.lib().sprg_mesh_last_stats(self.handle, st.ctypes.data, 7)
+        return dict(zip(["patches", "queued_exact", "expanded", "fallback", "status", "exact_tested",
+                         "large_sets"], st.tolist()))
 
     SHAPE_KEYS = ("fast_threads", "fast_blocks", "exact_blocks", "expand_blocks", "tile_blocks")
 

@@ -100,8 +100,9 @@ int sprg_mesh_phase_times(sprg_mesh* mesh, double* ms, int n);
 /* Patch routes of the last recovery on the mesh: [0] patches, [1] patches
  * queued for the exact rank test / expansion, [2] expanded sets fitted,
  * [3] patches refitted by the fast forms' FP64 fallback, [4] status bits,
- * [5] queued patches whose expansion needed the reference's rank test.
- * Writes min(n, 6) values and returns 6.  No reference counterpart. */
+ * [5] queued patches whose expansion needed the reference's rank test,
+ * [6] sets above 192 sample rows fitted by the large-set kernel.
+ * Writes min(n, 7) values and returns 7.  No reference counterpart. */
 int sprg_mesh_last_stats(sprg_mesh* mesh, int64_t* stats, int n);
 /* Chooses the launch shapes of the patch kernels for this mesh by timing
  * recoveries of `ip` in `form` (coordinate descent over block sizes and
@@ -201,6 +202,24 @@ int sprg_batch_size(sprg_batch* batch, int ip_order, int nc,
                     const double* const* ip, const uint64_t* target,
                     double adapt_ratio, double alpha, double beta,
                     double* const* size, int form, double* ms);
+
+/* ---- MDS .smb files (host; mds/mds_smb.c:562-606, apf::loadMdsPart) -----
+ * One part file ("name0.smb"), versions 0..6, uncompressed.  sprg_smb_counts
+ * reads the header: counts[8] = vertices, edges, triangles, quads, hexes,
+ * prisms, pyramids, tets (SMB type order); version may be NULL.
+ * sprg_smb_read fills, for a part of tetrahedra with a linear coordinate
+ * field, tet_verts [nt][4] and tet_edges [nt][6] in the order
+ * apf::Mesh::getDownward gives (MDS derives both from the tet's stored
+ * triangles, mds/mds.c:652-686), edge_verts [ne][2] and xyz [nv][3], with
+ * the file's entity indices; tet_edges and edge_verts may be NULL.  Parts
+ * holding other element types get SPRG_EUNSUPPORTED (the message lists
+ * them).  sprg_mesh_load_smb reads a part into a device mirror of
+ * coordinate order 1, or 2 with edge nodes at the edge midpoints (the
+ * projection of apf::changeShape to Lagrange-2). */
+int sprg_smb_counts(const char* path, int64_t* counts, int* version);
+int sprg_smb_read(const char* path, int32_t* tet_verts, int32_t* tet_edges,
+                  int32_t* edge_verts, double* xyz);
+int sprg_mesh_load_smb(sprg_ctx* ctx, const char* path, int order, sprg_mesh** out);
 
 /* ---- synthetic box meshes (host; mds/apfBox.cc:178-336 numbering) ------ */
 int sprg_box_counts(int n, int64_t* nv, int64_t* nt, int64_t* ne);

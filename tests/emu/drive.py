@@ -22,6 +22,9 @@ _lib.LIB_PATH = os.path.join(HERE, "_build", "libsprgpu_emu.so")
 from core_amd import mesh, workloads  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
+sys.path.insert(0, os.path.dirname(HERE))
+import smb_util  # noqa: E402
+
 L = _lib.lib()
 vp = ctypes.c_void_p
 
@@ -162,6 +165,10 @@ def main():
     ctx = vp()
     _lib.check(L.sprg_ctx_create(0, ctypes.byref(ctx)))
     exact_sequence(ctx, "p1 n=2 jittered", mesh.box(2, order=1, jitter=0.25, seed=11))
+    # one vertex holding 116 tets: 464 sample rows at 4 points per tet,
+    # fitted by the large-set kernel
+    tv, xyz = smb_util.star(60)
+    exact_sequence(ctx, "star", mesh.HostMesh(tv, xyz, 1), shapes=False)
     # the unjittered P1 box at 4 points per tet (every tet's samples collapse
     # to its centroid: most patches take the rank test and the expansion)
     # (EMU_FULL=1 adds its fast forms, a few minutes more)

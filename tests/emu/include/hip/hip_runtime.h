@@ -17,6 +17,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) ((const void*)&(x))
 
 struct dim3 {
   unsigned x, y, z;
@@ -150,6 +152,12 @@ hipError_t hipFree(void* p);
 hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t st = 0);
 hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t st = 0);
 hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind k);
+// device arrays are host arrays here: the symbol's address is the array's
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym)
+{
+  *p = const_cast<void*>(sym);
+  return hipSuccess;
+}
 struct emu_graph;
 typedef emu_graph* hipGraph_t;
 typedef emu_graph* hipGraphExec_t;
