@@ -17,7 +17,10 @@ and the exact FP64 form that reproduces the reference bit for bit) with
 their accuracy against the exact form; roofline = the dominant kernel (the
 patch fit over every patch, one launch) with the recovery's algorithmic HBM
 bytes over its GPU time (HIP events the library records on its stream);
-cpu_baseline = the FP64 CPU oracle on a bounded sample of the same workload.
+cpu_baseline = the FP64 CPU oracle's whole step on the host cores, RCB-
+partitioned with a two-layer ghost halo (one forked process per part), plus
+one part's share on one core; its recovered field is checked bit for bit
+against the GPU's FP64 form.
 """
 import argparse
 import json
@@ -67,83 +70,46 @@ def copy_probe(torch, nbytes=1 << 30, reps=5):
     return {"bytes_per_copy": 2 * nbytes, "best_ms": best, "gbs": 2 * nbytes / (best / 1e3) / 1e9}
 
 
-_CPU = {}
+def cpu_baseline(host, ip, workers):
+    """The FP64 oracle's whole step, partitioned: RCB parts with a two-layer
+    ghost halo, one forked process per part (oracle/partition.py), timed
+    start to end; then part 0's share alone on one core.  Returns the
+    record and the CPU run's size and recovered fields (checked against the
+    GPU's FP64 form later).  Call it before the GPU is touched: the parts
+    are forked."""
+    from oracle import partition
 
-
-def _oracle_timings(job):
-    """Worker: net oracle time of one chunk of vertex patches, edge patches
-    and tets (the upward-adjacency rebuild each oracle recover call starts
-    with, timed on an empty range, is mesh setup and is taken out)."""
-    from oracle import oracle as O
-
-    om, ip, rec = _CPU["om"], _CPU["ip"], _CPU["rec"]
-    (v0, v1), (e0, e1), (t0, t1) = job
-
-    def timed(fn, *a):
-        t = time.perf_counter()
-        fn(*a)
-        return time.perf_counter() - t
-
-    fixed = min(timed(O.recover, om, ip, 0, 0) for _ in range(2))
-    dv = max(timed(O.recover, om, ip, v0, v1) - fixed, 1e-9) if v1 > v0 else 0.0
-    de = max(timed(O.recover, om, ip, e0, e1) - fixed, 1e-9) if e1 > e0 else 0.0
-    dt = timed(O.element_error_range, om, 2, ip, rec, t0, t1) if t1 > t0 else 0.0
-    return dv, de, dt
-
-
-def cpu_baseline(host, ip, budget_s, workers):
-    """The FP64 oracle on a sample of patches and tets from the middle of the
-    workload, scaled to the whole step: serially on one core, then split over
-    `workers` processes (one per core, each on its own contiguous share, as a
-    partitioned run would be).  Call it before the GPU is touched: the
-    workers are forked."""
-    import multiprocessing
-
-    import numpy as np
-    from oracle import oracle as O
-
-    _CPU["om"] = O.Mesh(host.tv, host.xyz, host.order, te=host.te, ev=host.ev)
-    _CPU["ip"] = ip
-    _CPU["rec"] = np.zeros((host.nnodes, ip.shape[2]))
+    nproc, avail = partition.cores()
+    size, rec, info = partition.spr_size(host, ip, 2, 0.05, workers, serial_part=0)
     V, E, T = host.nv, host.ne, host.nt
-    mid_v, mid_e, mid_t = V // 2, V + E // 2, T // 2
-
-    def job(nv_s, ne_s, nt_s, k=0, parts=1):
-        def share(base, n):
-            a = base + n * k // parts
-            return a, base + n * (k + 1) // parts
-        return share(mid_v, nv_s), share(mid_e, ne_s), share(mid_t, nt_s)
-
-    dv, de, dt = _oracle_timings(job(50, 300, 5000))  # probe, then size the sample
-    tv, te, tt = dv / 50, de / 300, dt / 5000
-    frac = min(0.5, budget_s / (tv * V + te * E + tt * T))
-    nv_s, ne_s, nt_s = max(50, int(V * frac)), max(300, int(E * frac)), max(5000, int(T * frac))
-    nv_s, ne_s, nt_s = min(nv_s, V - mid_v), min(ne_s, E // 2), min(nt_s, T - mid_t)
-    dv, de, dt = _oracle_timings(job(nv_s, ne_s, nt_s))
-    tv, te, tt = dv / nv_s, de / ne_s, dt / nt_s
-    step_s = tv * V + te * E + tt * T
+    wall = info["wall_s"]
+    ph = info["phase_s"]
+    c = info["serial_counts"]
+    sp = c["vertices"] + c["edges"]
     out = {
-        "value": (V + E) / step_s,
+        "value": (V + E) / wall,
         "unit": "patches/s",
-        "cores": 1,
+        "cores": workers,
         "kind": "port",
-        "sample": ("FP64 oracle (reference operation order, explicit-Q Householder), serial: "
-                   "%d vertex patches + %d edge patches recovered and %d tets integrated from the "
-                   "middle of the mesh, per-item times scaled to V+E patches and T tets "
-                   "(est. full CPU step %.0f s; vertex patch %.3f ms, edge patch %.3f ms, "
-                   "tet %.2f us)" % (nv_s, ne_s, nt_s, step_s, tv * 1e3, te * 1e3, tt * 1e6)),
+        "nproc": nproc,
+        "cores_available": avail,
+        "step_s": wall,
+        "sample": ("the whole step, not a sample: getSPRSizeField(0.05) over all %d patches and %d "
+                   "tets with the FP64 oracle (reference operation order, explicit-Q Householder), "
+                   "RCB-partitioned into %d parts with a 2-layer ghost halo, one forked process per "
+                   "part; host nproc %d, %d cores available, %d used; %.2f s start to end "
+                   "(per part, barrier waits excluded: recovery %.2f-%.2f s, errors up to %.2f s, "
+                   "sizes up to %.3f s, average up to %.3f s)"
+                   % (V + E, T, workers, nproc, avail, workers, wall, ph[:, 0].min(), ph[:, 0].max(),
+                      ph[:, 1].max(), ph[:, 2].max(), ph[:, 3].max())),
+        "serial": {
+            "value": sp / info["serial_s"], "unit": "patches/s", "cores": 1, "kind": "port",
+            "sample": ("part 0 of the %d alone on one core, after the partitioned step: the step's "
+                       "work for its %d vertex + %d edge patches and %d tets (local mesh %d tets), "
+                       "%.2f s" % (workers, c["vertices"], c["edges"], c["tets"], c["local_tets"],
+                                   info["serial_s"]))},
     }
-    if workers > 1:
-        with multiprocessing.get_context("fork").Pool(workers) as pool:
-            res = pool.map(_oracle_timings, [job(nv_s, ne_s, nt_s, k, workers) for k in range(workers)])
-        slowest = max(sum(r) for r in res)
-        par_s = slowest * (V / nv_s)
-        out["parallel"] = {
-            "value": (V + E) / par_s, "unit": "patches/s", "cores": workers, "kind": "port",
-            "sample": ("the same sample split into %d contiguous shares, one forked process per "
-                       "core; slowest share %.2f s scaled to the whole step (est. %.1f s)"
-                       % (workers, slowest, par_s))}
-    return out
+    return out, size, rec
 
 
 def traffic_of(form):
@@ -164,8 +130,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", default="fp64f", choices=["fp64", "fp32r", "fp64f"])
     ap.add_argument("--n", type=int, default=120)
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-workers", type=int, default=None,
+                    help="processes of the partitioned CPU baseline (default: the cores available, "
+                         "at most 16, the host share of one GPU)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--exact-steps", type=int, default=2)
@@ -198,12 +165,16 @@ def main():
     ip = workloads.strain_ip(host, progress=lambda d, n: log("  strain IP field %d/%d tets" % (d, n))
                              if d % (1 << 23) < (1 << 20) or d == n else None)
     log("strain IP field %s (%.1fs)" % (ip.shape, time.time() - t0))
-    cpu = None
+    cpu = cpu_size = cpu_rec = None
     if not args.no_cpu:
-        # before the GPU is initialised: the parallel baseline forks
-        log("cpu baseline (budget %.0f s, %d workers)" % (args.cpu_budget, args.cpu_workers))
-        cpu = cpu_baseline(host, ip, args.cpu_budget, args.cpu_workers)
-        log("cpu baseline: %s" % cpu)
+        # before the GPU is initialised: the partitioned baseline forks
+        from oracle import partition
+
+        workers = args.cpu_workers or min(16, partition.cores()[1])
+        log("cpu baseline: whole step over %d parts" % workers)
+        t1 = time.time()
+        cpu, cpu_size, cpu_rec = cpu_baseline(host, ip, workers)
+        log("cpu baseline (%.1fs): %s" % (time.time() - t1, cpu))
     probe = copy_probe(torch)
     log("copy probe: %s" % probe)
     m = spr.Mesh(host)
@@ -292,6 +263,14 @@ def main():
         r64 = spr.recoverField(eps, form="fp64")
         d64 = r64.data.clone()
         m.destroyField(r64)
+        if cpu is not None:
+            # the partitioned CPU run against the GPU's FP64 form: the same
+            # recovered field bit for bit, the size field to summation order
+            same = bool(torch.equal(torch.from_numpy(cpu_rec).to(d64.device), d64))
+            srel = float((torch.from_numpy(cpu_size).to(a64.device) / a64 - 1).abs().max())
+            cpu["agrees_with_gpu_fp64"] = {"recovered_bitwise": same, "size_max_rel": srel}
+            log("cpu baseline vs GPU fp64: %s" % cpu["agrees_with_gpu_fp64"])
+            cpu_rec = cpu_size = None
         scale = d64.abs().amax(0)
         keep = scale > 0
         forms = {}

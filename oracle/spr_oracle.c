@@ -445,7 +445,7 @@ int om_edge_tets(const om_mesh* m, int32_t* off, int32_t* idx)
 /* ------------------------------------------------------------------------ */
 /* Patch recovery -- spr/sprRecoverField.cc. */
 
-typedef struct {
+typedef struct om_patcher {
   const om_mesh* m;
   int q, nterms, nc;
   int32_t *v2t_off, *v2t;
@@ -670,38 +670,74 @@ int om_patch_sets(const om_mesh* m, int q, int32_t* set_off, int32_t* set_idx,
 
 /* runSpr (:327-353): per component, solveFromQR then evalPolynomial at the
  * entity's node (m->getPoint). */
-int om_recover_range(const om_mesh* m, int q, int nc, const double* ip,
-                     double* nodal, int64_t p0, int64_t p1)
+/* recoverField's loop over the patches [p0, p1) (spr/sprRecoverField.cc:
+ * 283-302) with a patcher whose adjacency is built. */
+static int recover_with(patcher* P, const double* ip, double* nodal,
+                        int64_t p0, int64_t p1)
 {
-  patcher P;
-  int err = patcher_init(&P, m, q, nc);
-  if (err) { patcher_free(&P); return err; }
+  const om_mesh* m = P->m;
+  const int q = P->q, nc = P->nc;
+  int err = 0;
   unsigned char used[16];
-  int n = P.nterms;
+  int n = P->nterms;
   double* b = NULL;
   int64_t bcap = 0;
   int64_t np = num_patches(m);
   if (p1 > np) p1 = np;
   for (int64_t e = p0; e < p1 && !err; ++e) {
-    if ((err = build_patch(&P, e, used))) break;
-    int64_t mrows = P.set_n * q;
-    ensure_rows(&P, mrows);
-    qr_form_q((int)mrows, n, P.Q, P.V, used);
+    if ((err = build_patch(P, e, used))) break;
+    int64_t mrows = P->set_n * q;
+    ensure_rows(P, mrows);
+    qr_form_q((int)mrows, n, P->Q, P->V, used);
     if (mrows > bcap) { bcap = mrows; b = (double*)realloc(b, sizeof(double) * (size_t)bcap); }
     double node[3], terms[10], coeffs[10];
     for (int j = 0; j < 3; ++j) node[j] = m->xyz[3 * e + j];
     poly_terms(m->order, node, terms);
     for (int c = 0; c < nc; ++c) {
-      for (int64_t s = 0; s < P.set_n; ++s)
+      for (int64_t s = 0; s < P->set_n; ++s)
         for (int l = 0; l < q; ++l)
-          b[s * q + l] = ip[((size_t)P.set[s] * q + l) * nc + c];
-      om_solve_from_qr((int)mrows, n, P.Q, P.R, b, coeffs);
+          b[s * q + l] = ip[((size_t)P->set[s] * q + l) * nc + c];
+      om_solve_from_qr((int)mrows, n, P->Q, P->R, b, coeffs);
       double v = 0; /* coeffs * terms (mth/mthVector.h:104-110) */
       for (int i = 0; i < n; ++i) v += coeffs[i] * terms[i];
       nodal[e * nc + c] = v;
     }
   }
   free(b);
+  return err;
+}
+
+om_patcher* om_patcher_new(const om_mesh* m, int q, int nc)
+{
+  patcher* P = (patcher*)malloc(sizeof(patcher));
+  if (!P) return NULL;
+  if (patcher_init(P, m, q, nc)) {
+    patcher_free(P);
+    free(P);
+    return NULL;
+  }
+  return P;
+}
+
+int om_patcher_recover(om_patcher* P, const double* ip, double* nodal,
+                       int64_t p0, int64_t p1)
+{
+  return recover_with(P, ip, nodal, p0, p1);
+}
+
+void om_patcher_free(om_patcher* P)
+{
+  if (!P) return;
+  patcher_free(P);
+  free(P);
+}
+
+int om_recover_range(const om_mesh* m, int q, int nc, const double* ip,
+                     double* nodal, int64_t p0, int64_t p1)
+{
+  patcher P;
+  int err = patcher_init(&P, m, q, nc);
+  if (!err) err = recover_with(&P, ip, nodal, p0, p1);
   patcher_free(&P);
   return err;
 }
@@ -824,6 +860,17 @@ int om_current_size(const om_mesh* m, double* h)
   return 0;
 }
 
+void om_csr_mean(int64_t n, const int32_t* off, const int32_t* idx,
+                 const double* val, double* out)
+{
+  for (int64_t v = 0; v < n; ++v) {
+    double s = 0;
+    for (int32_t k = off[v]; k < off[v + 1]; ++k) s += val[idx[k]];
+    s /= (double)(off[v + 1] - off[v]);
+    out[v] = s;
+  }
+}
+
 /* averageToVertex (spr/sprEstimateError.cc:255-265; avgToVtx,
  * spr/sprEstimateTargetError.cc:180-190): mean of the element sizes around a
  * vertex.
@@ -834,12 +881,7 @@ static void average_to_vertex(const om_mesh* m, const double* esize,
                               const int32_t* off, const int32_t* idx,
                               double* size)
 {
-  for (int64_t v = 0; v < m->nv; ++v) {
-    double s = 0;
-    for (int32_t k = off[v]; k < off[v + 1]; ++k) s += esize[idx[k]];
-    s /= (double)(off[v + 1] - off[v]);
-    size[v] = s;
-  }
+  om_csr_mean(m->nv, off, idx, esize, size);
 }
 
 static int check_ip(const om_mesh* m, int ip_order)
@@ -849,21 +891,16 @@ static int check_ip(const om_mesh* m, int ip_order)
   return 0;
 }
 
-/* getSPRSizeField (spr/sprEstimateError.cc:301-321): recover, size factor,
- * element size, vertex average. */
-int om_spr_size_rec(const om_mesh* m, int ip_order, int nc, const double* ip,
-                    const double* rec, double adapt_ratio, double* size,
-                    double* diag)
+/* SelfProduct (spr/sprEstimateError.cc:98-123): the integral of the
+ * recovered field's square over the tets [t0, t1), summed in tet order. */
+static double self_product(const om_mesh* m, int ip_order, int nc,
+                           const double* rec, int64_t t0, int64_t t1)
 {
-  int err = check_ip(m, ip_order);
-  if (err) return err;
-  int q = om_rule_npoints(ip_order);
-  double d = 3, p = m->order;
-  /* SelfProduct (spr/sprEstimateError.cc:98-123) */
+  const int q = om_rule_npoints(ip_order);
   double self = 0;
   int nn = tet_nodes(m->order);
   double* data = (double*)malloc(sizeof(double) * 10 * 9);
-  for (int64_t t = 0; t < m->nt; ++t) {
+  for (int64_t t = t0; t < t1; ++t) {
     double x[10][3], N10[10], v[9];
     tet_coords(m, t, x);
     gather_rec(m, t, nc, rec, data);
@@ -881,6 +918,38 @@ int om_spr_size_rec(const om_mesh* m, int ip_order, int nc, const double* ip,
     }
   }
   free(data);
+  return self;
+}
+
+int om_self_product_range(const om_mesh* m, int ip_order, int nc,
+                          const double* rec, int64_t t0, int64_t t1,
+                          double* out)
+{
+  int err = check_ip(m, ip_order);
+  if (err) return err;
+  if (t1 > m->nt) t1 = m->nt;
+  *out = self_product(m, ip_order, nc, rec, t0, t1);
+  return 0;
+}
+
+int om_current_size_range(const om_mesh* m, double* h, int64_t t0, int64_t t1)
+{
+  if (t1 > m->nt) t1 = m->nt;
+  for (int64_t t = t0; t < t1; ++t) h[t] = current_size(m, t);
+  return 0;
+}
+
+/* getSPRSizeField (spr/sprEstimateError.cc:301-321): recover, size factor,
+ * element size, vertex average. */
+int om_spr_size_rec(const om_mesh* m, int ip_order, int nc, const double* ip,
+                    const double* rec, double adapt_ratio, double* size,
+                    double* diag)
+{
+  int err = check_ip(m, ip_order);
+  if (err) return err;
+  double d = 3, p = m->order;
+  /* SelfProduct (spr/sprEstimateError.cc:98-123) */
+  double self = self_product(m, ip_order, nc, rec, 0, m->nt);
   /* Error (spr/sprEstimateError.cc:171-185) and computeSizeFactor (:205-217) */
   double* s = (double*)malloc(sizeof(double) * (size_t)m->nt);
   double errsum = 0;

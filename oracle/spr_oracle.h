@@ -83,6 +83,15 @@ int om_recover(const om_mesh* m, int q, int nc, const double* ip,
 int om_recover_range(const om_mesh* m, int q, int nc, const double* ip,
                      double* nodal, int64_t p0, int64_t p1);
 
+/* A patch fitter whose upward adjacency is built once, for repeated range
+ * recoveries on one part (om_recover_range builds it on every call).
+ * om_patcher_new returns NULL on error (message in om_last_error()). */
+typedef struct om_patcher om_patcher;
+om_patcher* om_patcher_new(const om_mesh* m, int q, int nc);
+int om_patcher_recover(om_patcher* p, const double* ip, double* nodal,
+                       int64_t p0, int64_t p1);
+void om_patcher_free(om_patcher* p);
+
 /* spr::getSPRSizeField: ip field of integration order ip_order -> size[nv].
  * diag (optional, 4 doubles): ||eps*||^2, sum of errors, size_factor, 0. */
 int om_spr_size(const om_mesh* m, int ip_order, int nc, const double* ip,
@@ -112,6 +121,17 @@ int om_element_error_range(const om_mesh* m, int ip_order, int nc,
                            int64_t t0, int64_t t1);
 /* apf::measure of the six edges, max per tet (getCurrentSize). */
 int om_current_size(const om_mesh* m, double* h);
+/* The same for tets [t0, t1) only. */
+int om_current_size_range(const om_mesh* m, double* h, int64_t t0, int64_t t1);
+/* out[v] = mean of val[idx[k]] for k in [off[v], off[v+1]), summed in list
+ * order (the vertex average of the size estimators). */
+void om_csr_mean(int64_t n, const int32_t* off, const int32_t* idx,
+                 const double* val, double* out);
+/* SelfProduct of getSPRSizeField (the recovered field's square integrated)
+ * over tets [t0, t1), summed in tet order into *out. */
+int om_self_product_range(const om_mesh* m, int ip_order, int nc,
+                          const double* rec, int64_t t0, int64_t t1,
+                          double* out);
 
 #ifdef __cplusplus
 }
