@@ -43,7 +43,7 @@ __device__ __forceinline__ int64_t node_of(const MeshView& m, int32_t t, int n)
 // compile time (a P1 mesh fits 4 terms, a P2 mesh 10), so the rule point and
 // shape values stay in registers.
 template <int ORDER>
-__device__ void sample_point(const MeshView& m, int32_t t, int l, double* x)
+__device__ __forceinline__ void sample_point(const MeshView& m, int32_t t, int l, double* x)
 {
   constexpr int NN = ORDER == 1 ? 4 : 10;
   double xi[3], N[NN];
@@ -68,7 +68,7 @@ __device__ void sample_point(const MeshView& m, int32_t t, int l, double* x)
 
 // Rows i = gl, gl+LANES, ... of the monomial matrix A of the tet set S.
 template <int NT, int LANES>
-__device__ void build_rows(const MeshView& m, int q, const int32_t* S, int nrows,
+__device__ __forceinline__ void build_rows(const MeshView& m, int q, const int32_t* S, int nrows,
                            double* R, int gl)
 {
   for (int i = gl; i < nrows; i += LANES) {
@@ -79,20 +79,49 @@ __device__ void build_rows(const MeshView& m, int q, const int32_t* S, int nrows
   }
 }
 
+// 0 + prod(i0) + prod(i0 + 1) + ... + prod(i1 - 1), added one by one in
+// that order (the reference's accumulation loops).  Full blocks of 16 form
+// their products ahead of the additions, so that the loads behind them are
+// in flight together instead of each waiting in the chain of dependent
+// adds; the tail goes four, then one, at a time.  A chain usually runs on
+// one lane of the group, but every instruction of it costs the whole wave,
+// so the blocks carry no per-element guards.
+template <class F>
+__device__ __forceinline__ double seq_sum(int i0, int i1, F prod)
+{
+  double s = 0;
+  int b = i0;
+  for (; b + 16 <= i1; b += 16) {
+    double p[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) p[u] = prod(b + u);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s += p[u];
+  }
+  for (; b + 4 <= i1; b += 4) {
+    double p[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p[u] = prod(b + u);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s += p[u];
+  }
+  for (; b < i1; ++b) s += prod(b);
+  return s;
+}
+
 // Householder R recurrence of mth::decomposeQR restricted to what the solve
 // uses (get_reflector + reflect_columns on columns >= k, mth/mthQR.cc:19-63).
 // R is [m][NT] row-major, V is [NT][MS] (reflector k, zero outside k..m-1).
 // Returns false as soon as a column norm falls below 1e-10 (the reference's
 // rank test, mth/mthQR.cc:31).  Every lane of the block must call it.
 template <int NT, int LANES, int MS>
-__device__ bool r_recurrence(double* R, double* V, double* sDot, double* sNorm,
+__device__ __forceinline__ bool r_recurrence(double* R, double* V, double* sDot, double* sNorm,
                              int m, int gl, bool act)
 {
   bool ok = act;
   for (int k = 0; k < NT; ++k) {
     if (ok && gl == 0) {
-      double cnorm = 0;
-      for (int i = k; i < m; ++i) cnorm += R[i * NT + k] * R[i * NT + k];
+      const double cnorm = seq_sum(k, m, [&](int i) { return R[i * NT + k] * R[i * NT + k]; });
       sNorm[0] = sqrt(cnorm);
     }
     __syncthreads();
@@ -104,8 +133,8 @@ __device__ bool r_recurrence(double* R, double* V, double* sDot, double* sNorm,
     __syncthreads();
     if (ok && gl == 0) {
       V[k * MS + k] += sgn(R[k * NT + k]) * cn;
-      double rnorm = 0;
-      for (int i = k; i < m; ++i) rnorm += V[k * MS + i] * V[k * MS + i];
+      const double* v = V + k * MS;
+      const double rnorm = seq_sum(k, m, [&](int i) { return v[i] * v[i]; });
       sNorm[1] = sqrt(rnorm);
     }
     __syncthreads();
@@ -115,9 +144,8 @@ __device__ bool r_recurrence(double* R, double* V, double* sDot, double* sNorm,
     }
     __syncthreads();
     if (ok && gl >= k && gl < NT) {
-      double dot = 0;
-      for (int i = k; i < m; ++i) dot += R[i * NT + gl] * V[k * MS + i];
-      sDot[gl] = dot;
+      const double* v = V + k * MS;
+      sDot[gl] = seq_sum(k, m, [&](int i) { return R[i * NT + gl] * v[i]; });
     }
     __syncthreads();
     if (ok)
@@ -179,8 +207,18 @@ __global__ __launch_bounds__(256) void k_classify(FitArgs a, int nterms)
 // either (bit 31 set: an expanded-set id) -- the fast forms' refits.
 enum { FIT_REGULAR = 0, FIT_EXPANDED = 1, FIT_MIXED = 2 };
 
+// Waves per SIMD the fit of a bucket is compiled for: the kernel is bound by
+// the latency of its sequential sums, so the small buckets (most patches)
+// ask for more resident waves; the Q row of MMAX doubles per lane sets the
+// floor of the register budget.
+// (Only targets that compile without scratch spills are requested.)
+constexpr int exact_waves(int nt, int mmax)
+{
+  return nt == 4 ? (mmax == 32 ? 2 : 1) : (mmax <= 32 ? 4 : mmax == 96 ? 2 : 1);
+}
+
 template <int NT, int MMAX, int LANES, int MODE>
-__global__ __launch_bounds__(64) void k_exact_fit(FitArgs a, int bucket)
+__global__ __launch_bounds__(64, exact_waves(NT, MMAX)) void k_exact_fit(FitArgs a, int bucket)
 {
   constexpr bool EXP = MODE == FIT_EXPANDED;
   constexpr int G = 64 / LANES;
@@ -249,6 +287,8 @@ __global__ __launch_bounds__(64) void k_exact_fit(FitArgs a, int bucket)
 #pragma unroll
         for (int l = 0; l < MMAX; ++l) dot += qr[l] * v[l];
         const double t2 = 2 * dot;
+        // read v again from LDS rather than keeping MMAX more registers live
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int l = 0; l < MMAX; ++l) qr[l] -= t2 * v[l];
       }
@@ -269,13 +309,10 @@ __global__ __launch_bounds__(64) void k_exact_fit(FitArgs a, int bucket)
     if (ok)
       for (int w = gl; w < NT * nc; w += LANES) {
         int i = w % NT, c = w / NT;
-        double y = 0;
-        for (int r = 0; r < mrows; ++r) {
-          double b = staged ? sV[g][r * nc + c]
-                            : a.ip[((int64_t)sS[g][r / q] * q + r % q) * nc + c];
-          y += sR[g][r * NT + i] * b;
-        }
-        sY[g][c * NT + i] = y;
+        sY[g][c * NT + i] = seq_sum(0, mrows, [&](int r) {
+          const double b = staged ? sV[g][r * nc + c] : a.ip[((int64_t)sS[g][r / q] * q + r % q) * nc + c];
+          return sR[g][r * NT + i] * b;
+        });
       }
     __syncthreads();
     // backsubUT (mth/mthQR.cc:131-148) and evalPolynomial at the node
