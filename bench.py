@@ -116,7 +116,7 @@ def traffic_of(form):
     """HBM bytes per launch of the fit pass from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE measurement (scaled by the copy probe), for the
     form it was measured on; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "k_fast_strain120_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_tile_strain120_traffic.json")
     if form != "fp64f" or not os.path.exists(path):
         return None
     with open(path) as f:
@@ -250,7 +250,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(args.form),
                      "measured_copy_gbs": probe["gbs"],
-                     "kernel": ("k_fast regular pass" if args.form != "fp64" else "exact fit")
+                     "kernel": ("k_tile regular pass" if args.form != "fp64" else "exact fit")
                                + " (all %d patches), %d algorithmic bytes per launch, %.3f ms"
                                % (patches, ab["recover"], fit_ms)},
     }

@@ -37,7 +37,6 @@
 namespace sprg {
 namespace {
 
-constexpr int XCD_K_TILE = 4;  // xcd_block chunk: neighbouring tiles share tets
 constexpr int SORT_CAP = 2048; // candidate tets a tile build sorts (sum of valences)
 
 // ---------------------------------------------------------------------------
@@ -169,6 +168,27 @@ __device__ __forceinline__ void sample_x(const MeshView& m, int32_t t, int l, do
     for (int j = 0; j < 3; ++j)
       x[j] = fma(xi[2], p3[j], fma(xi[1], p2[j], fma(xi[0], p1[j], l0 * p0[j])));
   }
+}
+
+// The tiles a block fits: the tiles are cut into eight contiguous ranges,
+// one per XCD (the dispatcher hands block b to XCD b mod 8), and the blocks
+// of an XCD take its range's tiles in turn.  Consecutive tiles hold
+// neighbouring vertices, so the tiles an XCD has in flight at once form a
+// compact region of the mesh whose tets its L2 keeps for the neighbouring
+// tiles that share them.  Every tile has a block when the grid has at least
+// eight blocks (the launch guarantees it).
+struct TileWalk {
+  int64_t first, end, step;
+};
+
+__device__ __forceinline__ TileWalk tile_walk(int64_t ntile)
+{
+  const int64_t g = gridDim.x, b = blockIdx.x, x = b % 8;
+  TileWalk w;
+  w.first = ntile * x / 8 + b / 8;
+  w.end = ntile * (x + 1) / 8;
+  w.step = (g - x + 7) / 8;  // the blocks b' < g with b' mod 8 == x
+  return w;
 }
 
 // What a lane needs to start a tile, loaded while the previous tile is
@@ -333,15 +353,15 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
   double* const sX = smem;                                                         // [umax*q][3]
   int32_t* const sTet = reinterpret_cast<int32_t*>(smem + 3 * tv.max_union * q);  // [umax]
   constexpr int NG = NT * (NT + 1) / 2;
-  int64_t tile = xcd_block(blockIdx.x, gridDim.x, XCD_K_TILE);
+  const TileWalk w = tile_walk(tv.ntile);
   TileNext nx;
-  next_meta(tv, tile, nx);
+  next_meta(tv, w.first < w.end ? w.first : tv.ntile, nx);
   next_ids(tv, nx);
   next_verts(m, nx);
-  for (; tile < tv.ntile; tile += gridDim.x) {
+  for (int64_t tile = w.first; tile < w.end; tile += w.step) {
     const TileNext cur = nx;
     stage_tets<P2MAP>(m, cur, q, sTet, sX);
-    next_meta(tv, tile + gridDim.x, nx);  // arrives during the Gram sums
+    next_meta(tv, tile + w.step < w.end ? tile + w.step : tv.ntile, nx);  // arrives during the Gram sums
     __syncthreads();
     // this lane's slot: patch, group size, rank in the group, set
     const bool has = (int)threadIdx.x < cur.nslot;
@@ -543,7 +563,8 @@ int launch_tile_nc(const FitArgs& a, float tau_rank, float tau_acc, hipStream_t 
 {
   const TileView& tv = a.tiles;
   const size_t lds = sizeof(double) * 3 * (size_t)tv.max_union * a.q + sizeof(int32_t) * tv.max_union;
-  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(a.shape.tile_blocks, tv.ntile));
+  // at least 8 blocks: tile_walk gives each XCD's range to its own blocks
+  const unsigned blocks = (unsigned)std::max<int64_t>(8, std::min<int64_t>(a.shape.tile_blocks, tv.ntile));
   switch (a.nc) {
     case 1: k_tile<T, NT, 1, NREF, P2><<<blocks, TILE_B, lds, s>>>(a, tau_rank, tau_acc); break;
     case 3: k_tile<T, NT, 3, NREF, P2><<<blocks, TILE_B, lds, s>>>(a, tau_rank, tau_acc); break;

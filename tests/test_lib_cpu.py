@@ -55,6 +55,19 @@ def test_box_counts(n):
     assert len({tuple(sorted(p)) for p in ev.tolist()}) == ne
 
 
+@pytest.mark.parametrize("n", [4, 64])
+def test_box_valence_histogram(n):
+    """Vertex valences of the Kuhn box (the graded n=64 workload's topology:
+    jitter and grading move vertices only): 24 tets around an interior vertex,
+    12 on a face, 4 or 8 on an edge (the diagonal's orientation), 2 or 6 at a
+    corner -- so no vertex patch of these meshes exceeds 24 tets (96 rows at
+    4 points per tet)."""
+    m, _ = workloads.graded_vec(n) if n == 64 else (mesh.box(n, order=2, jitter=0.3, seed=3, grade=True), None)
+    got = dict(zip(*np.unique(np.bincount(m.tv.ravel(), minlength=m.nv), return_counts=True)))
+    k = n - 1
+    assert {int(a): int(b) for a, b in got.items()} == {24: k ** 3, 12: 6 * k * k, 8: 6 * k, 4: 6 * k, 6: 2, 2: 6}
+
+
 def test_box_numbering_matches_makeMdsBox():
     n = 3
     h = mesh.box(n, order=2)
