@@ -279,8 +279,9 @@ template <int ST>
 __device__ __forceinline__ int xor_lane(int x)
 {
   static_assert(ST == 1 || ST == 2 || ST == 4, "lane exchange distance");
-  if constexpr (ST == 1) return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
-  if constexpr (ST == 2) return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+  // every lane reads a lane of its own quad, so no lane needs an old value
+  if constexpr (ST == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true);  // quad_perm 1,0,3,2
+  if constexpr (ST == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, true);  // quad_perm 2,3,0,1
   return __builtin_amdgcn_ds_swizzle(x, 0x101F);                                          // xor 4 in 32 lanes
 }
 
@@ -297,15 +298,17 @@ __device__ __forceinline__ float xor_lane(float x)
   return __int_as_float(xor_lane<ST>(__float_as_int(x)));
 }
 
+// One butterfly level: lanes of groups wider than ST add their partner's
+// value, the others add it times zero.  The partner of a lane outside such a
+// group belongs to another group or to no patch, whose sums are finite (or
+// the zeros they start from), so the product is +-0 and the value stays;
+// inside a group, t * 1 + v rounds once, as v + t does.
 template <int ST, class T, int N>
 __device__ __forceinline__ void group_step(T* v, int g)
 {
-  const bool in = g > ST;
+  const T w = g > ST ? (T)1 : (T)0;
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    const T t = xor_lane<ST>(v[k]);
-    v[k] = in ? v[k] + t : v[k];
-  }
+  for (int k = 0; k < N; ++k) v[k] = fma(xor_lane<ST>(v[k]), w, v[k]);
 }
 
 // Sums v over the lanes of each group (g lanes, aligned); every lane of a
@@ -318,6 +321,47 @@ __device__ __forceinline__ void group_sum(T* v, int g, int gmax)
   if (gmax > 1) group_step<1, T, N>(v, g);
   if (gmax > 2) group_step<2, T, N>(v, g);
   if (gmax > 4) group_step<4, T, N>(v, g);
+}
+
+// 1 / sqrt(d) for d > 0: the hardware estimate and Newton steps
+// r <- r + r (1/2 - (d/2) r^2), two in FP64 (each doubles the correct bits)
+// and one in FP32.
+__device__ __forceinline__ double rsqrt_nr(double d)
+{
+  double r = __builtin_amdgcn_rsq(d);
+  const double h = 0.5 * d;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) r = fma(r, fma(-h, r * r, 0.5), r);
+  return r;
+}
+
+__device__ __forceinline__ float rsqrt_nr(float d)
+{
+  const float r = __builtin_amdgcn_rsqf(d);
+  return fmaf(r, fmaf(-0.5f * d, r * r, 0.5f), r);
+}
+
+// body(u) for the lane's tets s = rank, rank + g, ... < ns of its patch set
+// (u: the tet's position in the tile's union), the next position loaded
+// while the current tet is processed.
+template <class F>
+__device__ __forceinline__ void for_lane_tets(const uint16_t* loc, int rank, int g, int ns, F body)
+{
+  if (rank >= ns) return;
+  int u = loc[rank];
+  for (int s = rank; s < ns; s += g) {
+    const int un = loc[s + g < ns ? s + g : ns - 1];
+    body(u);
+    u = un;
+  }
+}
+
+// body(l) for the q points of a tet (a rolled loop: unrolled, the points'
+// loads would be hoisted next to the 55 Gram accumulators and spill them).
+template <class F>
+__device__ __forceinline__ void for_points(int q, F body)
+{
+  for (int l = 0; l < q; ++l) body(l);
 }
 
 // Forward and back substitution with the Cholesky factor whose diagonal
@@ -387,9 +431,9 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
 #pragma unroll
     for (int k = 0; k < NG; ++k) G[k] = 0;
     if (work)
-      for (int s = rank; s < ns; s += g) {
-        const double* X = sX + 3 * q * (int)loc[s];
-        for (int l = 0; l < q; ++l) {
+      for_lane_tets(loc, rank, g, ns, [&](int u) {
+        const double* X = sX + 3 * q * u;
+        for_points(q, [&](int l) {
           const double d[3] = {X[3 * l] - xn[0], X[3 * l + 1] - xn[1], X[3 * l + 2] - xn[2]};
           T f[NT];
           basis<T, NT>(d, f);
@@ -397,8 +441,8 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
           for (int i = 0; i < NT; ++i)
 #pragma unroll
             for (int j = 0; j <= i; ++j) G[tri(i, j)] = fma(f[i], f[j], G[tri(i, j)]);
-        }
-      }
+        });
+      });
     next_ids(tv, nx);  // arrives during the reduction and the factorisation
     group_sum<T, NG>(G, g, gmax);
     // Cholesky in place, with the rank certificate and the conditioning
@@ -413,9 +457,9 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
         for (int j = 0; j < k; ++j) dk = fma(-G[tri(k, j)], G[tri(k, j)], dk);
         const float ratio = (diag > 0 && dk > 0) ? (float)(dk / diag) : 0.0f;
         min_ratio = fminf(min_ratio, ratio);
-        const T lkk = dk > 0 ? sqrt(dk) : (T)0;
-        if (!(ratio >= tau_rank) || !((double)lkk >= 1.05e-10)) rank_ok = false;
-        const T rk = lkk > 0 ? (T)1 / lkk : (T)1;  // the diagonal keeps 1 / L_kk
+        // L_kk = sqrt(dk) must clear 1.05e-10: dk >= 1.05e-10^2
+        if (!(ratio >= tau_rank) || !((double)dk >= 1.1025e-20)) rank_ok = false;
+        const T rk = dk > 0 ? rsqrt_nr(dk) : (T)1;  // the diagonal keeps 1 / L_kk
         G[tri(k, k)] = rk;
 #pragma unroll
         for (int i = k + 1; i < NT; ++i) {
@@ -463,9 +507,9 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
 #pragma unroll
         for (int k = 0; k < NT; ++k) u[k] = 0;
         if (refine)
-          for (int s = rank; s < ns; s += g) {
-            const double* X = sX + 3 * q * (int)loc[s];
-            for (int l = 0; l < q; ++l) {
+          for_lane_tets(loc, rank, g, ns, [&](int ut) {
+            const double* X = sX + 3 * q * ut;
+            for_points(q, [&](int l) {
               const double d[3] = {X[3 * l] - xn[0], X[3 * l + 1] - xn[1], X[3 * l + 2] - xn[2]};
               double f[NT];
               basis<double, NT>(d, f);
@@ -474,8 +518,8 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
               for (int k = 0; k < NT; ++k) t = fma(f[k], z[k], t);
 #pragma unroll
               for (int k = 0; k < NT; ++k) u[k] = fma(f[k], t, u[k]);
-            }
-          }
+            });
+          });
         group_sum<double, NT>(u, g, gmax);
         if (refine) {
           T res[NT], dz[NT];
@@ -524,8 +568,7 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
       };
       if (q == 4 && 4 * NC <= 36) {
         // the tet's whole q x nc block in flight at once (G is dead here)
-        for (int s = rank; s < ns; s += g) {
-          const int u = loc[s];
+        for_lane_tets(loc, rank, g, ns, [&](int u) {
           const double* X = sX + 12 * u;
           const double* bt = a.ip + (int64_t)sTet[u] * 4 * NC;
           double b[4 * NC];
@@ -533,10 +576,9 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
           for (int i = 0; i < 4 * NC; ++i) b[i] = bt[i];
 #pragma unroll
           for (int l = 0; l < 4; ++l) accumulate(X + 3 * l, b + l * NC);
-        }
+        });
       } else {
-        for (int s = rank; s < ns; s += g) {
-          const int u = loc[s];
+        for_lane_tets(loc, rank, g, ns, [&](int u) {
           const double* X = sX + 3 * q * u;
           const double* bt = a.ip + (int64_t)sTet[u] * q * NC;
           for (int l = 0; l < q; ++l) {
@@ -545,7 +587,7 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
             for (int c = 0; c < NC; ++c) b[c] = bt[l * NC + c];
             accumulate(X + 3 * l, b);
           }
-        }
+        });
       }
     }
     group_sum<double, NC>(v, g, gmax);

@@ -10,6 +10,7 @@ non-zero on the first failure."""
 import ctypes
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -39,11 +40,18 @@ def check(name, ok):
         sys.exit(1)
 
 
+# persistent grids of a few blocks: the emulator walks every block of a
+# grid, so the default grids (thousands of blocks) only cost time on these
+# meshes, and small grids give every block several work items
+SMALL_SHAPE = (ctypes.c_int64 * 5)(64, 24, 16, 8, 12)
+
+
 def device_mesh(ctx, h):
     m = vp()
     _lib.check(L.sprg_mesh_create(ctx, h.nv, h.nt, h.ne if h.te is not None else 0, P(h.tv),
                                   None if h.te is None else P(h.te), P(h.xyz), h.order,
                                   ctypes.byref(m)))
+    _lib.check(L.sprg_mesh_set_shape(m, SMALL_SHAPE, 5))
     return m
 
 
@@ -164,17 +172,27 @@ def batch_check(ctx):
 def main():
     ctx = vp()
     _lib.check(L.sprg_ctx_create(0, ctypes.byref(ctx)))
-    exact_sequence(ctx, "p1 n=2 jittered", mesh.box(2, order=1, jitter=0.25, seed=11))
+    t0 = time.time()
+
+    def timed(label, fn, *args, **kw):
+        t = time.time()
+        fn(*args, **kw)
+        print("TIME %s %.1f s" % (label, time.time() - t), flush=True)
+
+    timed("p1 n=2", exact_sequence, ctx, "p1 n=2 jittered", mesh.box(2, order=1, jitter=0.25, seed=11))
+    timed("p2 n=2", exact_sequence, ctx, "p2 n=2 jittered", mesh.box(2, order=2, jitter=0.25, seed=12),
+          shapes=False)
     # one vertex holding 116 tets: 464 sample rows at 4 points per tet,
     # fitted by the large-set kernel
     tv, xyz = smb_util.star(60)
-    exact_sequence(ctx, "star", mesh.HostMesh(tv, xyz, 1), shapes=False)
+    timed("star", exact_sequence, ctx, "star", mesh.HostMesh(tv, xyz, 1), shapes=False)
     # the unjittered P1 box at 4 points per tet (every tet's samples collapse
     # to its centroid: most patches take the rank test and the expansion)
     # (EMU_FULL=1 adds its fast forms, a few minutes more)
-    exact_sequence(ctx, "p1 n=4", mesh.box(4, order=1), shapes=False, fast=bool(os.environ.get("EMU_FULL")))
-    batch_check(ctx)
-    print("ALL PASS", flush=True)
+    timed("p1 n=4", exact_sequence, ctx, "p1 n=4", mesh.box(4, order=1), shapes=False,
+          fast=bool(os.environ.get("EMU_FULL")))
+    timed("batch", batch_check, ctx)
+    print("ALL PASS (%.0f s)" % (time.time() - t0), flush=True)
 
 
 if __name__ == "__main__":

@@ -85,6 +85,14 @@ inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask,
   const int from = (l & ~3) | ((ctrl >> (2 * (l & 3))) & 3);
   return (int)emu::shfl64(src, from);
 }
+inline int __builtin_amdgcn_mov_dpp(int src, int ctrl, int row_mask, int bank_mask, bool bound)
+{
+  return __builtin_amdgcn_update_dpp(0, src, ctrl, row_mask, bank_mask, bound);
+}
+// the hardware reciprocal square root estimates, exact here (the callers
+// refine them with Newton steps)
+inline double __builtin_amdgcn_rsq(double x) { return 1.0 / std::sqrt(x); }
+inline float __builtin_amdgcn_rsqf(float x) { return 1.0f / std::sqrt(x); }
 // ds_swizzle in bitmask mode (offset bit 15 clear): within groups of 32
 // lanes, lane l reads ((l & and) | or) ^ xor
 inline int __builtin_amdgcn_ds_swizzle(int src, int pattern)
