@@ -426,25 +426,44 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
       xn[1] = m.xyz[3 * p + 1];
       xn[2] = m.xyz[3 * p + 2];
     }
-    // Gram matrix of the lane's tets, packed lower triangle
+    // Gram matrix of the group's tets, packed lower triangle
     T G[NG];
+    if constexpr (NT == 10) {
+      // from the moments of the lane's rows, summed over the group
+      T M[NMOM];
 #pragma unroll
-    for (int k = 0; k < NG; ++k) G[k] = 0;
-    if (work)
-      for_lane_tets(loc, rank, g, ns, [&](int u) {
-        const double* X = sX + 3 * q * u;
-        for_points(q, [&](int l) {
-          const double d[3] = {X[3 * l] - xn[0], X[3 * l + 1] - xn[1], X[3 * l + 2] - xn[2]};
-          T f[NT];
-          basis<T, NT>(d, f);
-#pragma unroll
-          for (int i = 0; i < NT; ++i)
-#pragma unroll
-            for (int j = 0; j <= i; ++j) G[tri(i, j)] = fma(f[i], f[j], G[tri(i, j)]);
+      for (int k = 0; k < NMOM; ++k) M[k] = 0;
+      if (work)
+        for_lane_tets(loc, rank, g, ns, [&](int u) {
+          const double* X = sX + 3 * q * u;
+          for_points(q, [&](int l) {
+            const double d[3] = {X[3 * l] - xn[0], X[3 * l + 1] - xn[1], X[3 * l + 2] - xn[2]};
+            moments_row<T>(d, M);
+          });
         });
-      });
-    next_ids(tv, nx);  // arrives during the reduction and the factorisation
-    group_sum<T, NG>(G, g, gmax);
+      next_ids(tv, nx);  // arrives during the reduction and the factorisation
+      group_sum<T, NMOM>(M, g, gmax);
+      M[0] = (T)(ns * q);
+      gram_from_moments<T>(M, G);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NG; ++k) G[k] = 0;
+      if (work)
+        for_lane_tets(loc, rank, g, ns, [&](int u) {
+          const double* X = sX + 3 * q * u;
+          for_points(q, [&](int l) {
+            const double d[3] = {X[3 * l] - xn[0], X[3 * l + 1] - xn[1], X[3 * l + 2] - xn[2]};
+            T f[NT];
+            basis<T, NT>(d, f);
+#pragma unroll
+            for (int i = 0; i < NT; ++i)
+#pragma unroll
+              for (int j = 0; j <= i; ++j) G[tri(i, j)] = fma(f[i], f[j], G[tri(i, j)]);
+          });
+        });
+      next_ids(tv, nx);  // arrives during the reduction and the factorisation
+      group_sum<T, NG>(G, g, gmax);
+    }
     // Cholesky in place, with the rank certificate and the conditioning
     bool rank_ok = true;
     float min_ratio = 1.0f;

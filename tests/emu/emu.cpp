@@ -1,11 +1,18 @@
 // Launch machinery of the CPU emulation (see include/hip/hip_runtime.h).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
-#include <condition_variable>
+#include <climits>
+#include <cstdint>
 #include <mutex>
 #include <thread>
 #include <vector>
+
+#include <linux/futex.h>
+#include <sys/syscall.h>
+#include <time.h>
+#include <unistd.h>
 
 thread_local dim3 threadIdx, blockIdx;
 
@@ -14,26 +21,41 @@ dim3 blockDim, gridDim;
 namespace emu {
 namespace {
 
+// A barrier of n threads: an arrival counter and a generation word.  The
+// last thread to arrive resets the counter, bumps the generation and wakes
+// the sleepers with one futex call; the others spin briefly, then sleep on
+// the generation word (no mutex for every woken thread to take in turn).
 struct Barrier {
-  std::mutex mu;
-  std::condition_variable cv;
-  int n = 0, waiting = 0;
-  uint64_t gen = 0;
-  void reset(int count) { n = count; waiting = 0; }
+  std::atomic<int> count{0};
+  std::atomic<uint32_t> gen{0};
+  int n = 0;
+  void reset(int c)
+  {
+    n = c;
+    count.store(0);
+  }
   void wait(const char* what)
   {
-    std::unique_lock<std::mutex> lk(mu);
-    uint64_t g = gen;
-    if (++waiting == n) {
-      waiting = 0;
-      ++gen;
-      cv.notify_all();
+    const uint32_t g = gen.load(std::memory_order_acquire);
+    if (count.fetch_add(1, std::memory_order_acq_rel) + 1 == n) {
+      count.store(0, std::memory_order_relaxed);
+      gen.fetch_add(1, std::memory_order_acq_rel);
+      syscall(SYS_futex, reinterpret_cast<uint32_t*>(&gen), FUTEX_WAKE_PRIVATE, INT32_MAX, nullptr, nullptr, 0);
       return;
     }
-    if (!cv.wait_for(lk, std::chrono::seconds(20), [&] { return gen != g; })) {
-      fprintf(stderr, "EMU DEADLOCK at %s (block %u thread %u, %d of %d waiting)\n", what,
-              blockIdx.x, threadIdx.x, waiting, n);
-      abort();
+    for (int i = 0; i < 64; ++i) {
+      if (gen.load(std::memory_order_acquire) != g) return;
+      std::this_thread::yield();
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    while (gen.load(std::memory_order_acquire) == g) {
+      struct timespec ts = {1, 0};
+      syscall(SYS_futex, reinterpret_cast<uint32_t*>(&gen), FUTEX_WAIT_PRIVATE, g, &ts, nullptr, 0);
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) {
+        fprintf(stderr, "EMU DEADLOCK at %s (block %u thread %u, %d of %d waiting)\n", what,
+                blockIdx.x, threadIdx.x, count.load(), n);
+        abort();
+      }
     }
   }
 };
