@@ -532,14 +532,22 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
         for (int i = lane; i < HSIZE; i += 64) sHash[i] = -1;
         if (lane == 0) sN = 0;
         __syncthreads();
-        for (int w = lane; w < nold * 4; w += 64) {
+        // items: (tet of the old set, one of its vertices); with P >= the
+        // item count (a power of two, at most 64) each item gets R = 64 / P
+        // lanes, which take its neighbour list's chunks of 8 in turn (the
+        // same candidate pairs as one lane per item, walked in parallel)
+        const int nitem = nold * 4;
+        int P = 64;
+        while (P > 1 && P / 2 >= nitem) P /= 2;
+        const int R = 64 / P, sub = lane / P;
+        for (int w = lane % P; w < nitem; w += P) {
           const int32_t t = sOld[w >> 2];
           const int32_t* tv = m.tv + 4 * (int64_t)t;
           const int32_t v = tv[w & 3];
           const int32_t k0 = m.v2t_off[v], k1 = m.v2t_off[v + 1];
           // neighbour ids and their vertices eight at a time, so the
           // dependent loads overlap
-          for (int32_t k = k0; k < k1; k += 8) {
+          for (int32_t k = k0 + 8 * sub; k < k1; k += 8 * R) {
             int32_t u[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) u[j] = (k + j < k1) ? m.v2t[k + j] : -1;
