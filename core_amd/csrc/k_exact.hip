@@ -79,17 +79,16 @@ __device__ __forceinline__ void build_rows(const MeshView& m, int q, const int32
   }
 }
 
-// 0 + prod(i0) + prod(i0 + 1) + ... + prod(i1 - 1), added one by one in
-// that order (the reference's accumulation loops).  Full blocks of 16 form
+// s + prod(i0) + prod(i0 + 1) + ... + prod(i1 - 1) (s = 0 unless given),
+// added one by one in that order (the reference's accumulation loops).  Full blocks of 16 form
 // their products ahead of the additions, so that the loads behind them are
 // in flight together instead of each waiting in the chain of dependent
 // adds; the tail goes four, then one, at a time.  A chain usually runs on
 // one lane of the group, but every instruction of it costs the whole wave,
 // so the blocks carry no per-element guards.
 template <class F>
-__device__ __forceinline__ double seq_sum(int i0, int i1, F prod)
+__device__ __forceinline__ double seq_sum(int i0, int i1, F prod, double s = 0)
 {
-  double s = 0;
   int b = i0;
   for (; b + 16 <= i1; b += 16) {
     double p[16];
@@ -109,6 +108,33 @@ __device__ __forceinline__ double seq_sum(int i0, int i1, F prod)
   return s;
 }
 
+// Two independent sums of the form of seq_sum over the same range, their
+// additions interleaved so that each chain's latency hides the other's.
+template <class F0, class F1>
+__device__ __forceinline__ void seq_sum2(int i0, int i1, F0 prod0, F1 prod1, double& s0, double& s1)
+{
+  s0 = 0;
+  s1 = 0;
+  int b = i0;
+  for (; b + 8 <= i1; b += 8) {
+    double p0[8], p1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      p0[u] = prod0(b + u);
+      p1[u] = prod1(b + u);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      s0 += p0[u];
+      s1 += p1[u];
+    }
+  }
+  for (; b < i1; ++b) {
+    s0 += prod0(b);
+    s1 += prod1(b);
+  }
+}
+
 // Householder R recurrence of mth::decomposeQR restricted to what the solve
 // uses (get_reflector + reflect_columns on columns >= k, mth/mthQR.cc:19-63).
 // R is [m][NT] row-major, V is [NT][MS] (reflector k, zero outside k..m-1).
@@ -120,27 +146,27 @@ __device__ __forceinline__ bool r_recurrence(double* R, double* V, double* sDot,
 {
   bool ok = act;
   for (int k = 0; k < NT; ++k) {
+    // get_reflector's two norms on one lane: cnorm over column k, then
+    // rnorm over v = column k with v_k += sign(R_kk) cnorm, summed from
+    // v_k^2 (0 + v_k^2 is v_k^2) on down the column, as the reference
+    // sums the copied vector
     if (ok && gl == 0) {
-      const double cnorm = seq_sum(k, m, [&](int i) { return R[i * NT + k] * R[i * NT + k]; });
-      sNorm[0] = sqrt(cnorm);
+      const double cn = sqrt(seq_sum(k, m, [&](int i) { return R[i * NT + k] * R[i * NT + k]; }));
+      sNorm[0] = cn;
+      if (!(cn < 1e-10)) {
+        const double vk = R[k * NT + k] + sgn(R[k * NT + k]) * cn;
+        sNorm[2] = vk;
+        sNorm[1] = sqrt(seq_sum(k + 1, m, [&](int i) { return R[i * NT + k] * R[i * NT + k]; }, vk * vk));
+      }
     }
     __syncthreads();
-    double cn = ok ? sNorm[0] : 0;
+    const double cn = ok ? sNorm[0] : 0;
     if (ok && cn < 1e-10) ok = false;
-    if (ok)
-      for (int i = gl; i < MS; i += LANES)
-        V[k * MS + i] = (i >= k && i < m) ? R[i * NT + k] : 0.0;
-    __syncthreads();
-    if (ok && gl == 0) {
-      V[k * MS + k] += sgn(R[k * NT + k]) * cn;
-      const double* v = V + k * MS;
-      const double rnorm = seq_sum(k, m, [&](int i) { return v[i] * v[i]; });
-      sNorm[1] = sqrt(rnorm);
-    }
-    __syncthreads();
     if (ok) {
-      double rn = sNorm[1];
-      for (int i = k + gl; i < m; i += LANES) V[k * MS + i] /= rn;
+      // the normalised reflector, zero outside rows k..m-1
+      const double rn = sNorm[1], vk = sNorm[2];
+      for (int i = gl; i < MS; i += LANES)
+        V[k * MS + i] = (i >= k && i < m) ? (i == k ? vk : R[i * NT + k]) / rn : 0.0;
     }
     __syncthreads();
     if (ok && gl >= k && gl < NT) {
@@ -227,7 +253,7 @@ __global__ __launch_bounds__(64, exact_waves(NT, MMAX)) void k_exact_fit(FitArgs
   __shared__ double sRt[G][NT * NT];
   __shared__ double sY[G][27 * NT];
   __shared__ double sDot[G][NT];
-  __shared__ double sNorm[G][2];
+  __shared__ double sNorm[G][3];
   __shared__ int32_t sS[G][MMAX];
   const int g = threadIdx.x / LANES, gl = threadIdx.x % LANES;
   const MeshView& m = a.m;
@@ -300,19 +326,30 @@ __global__ __launch_bounds__(64, exact_waves(NT, MMAX)) void k_exact_fit(FitArgs
     // stage b = IP values of the patch rows in sV when it fits
     const bool staged = nc <= NT;
     if (ok && staged)
-      for (int i = gl; i < mrows * nc; i += LANES) {
-        int r = i / nc, c = i % nc;
-        sV[g][i] = a.ip[((int64_t)sS[g][r / q] * q + r % q) * nc + c];
+      // a row per lane: its nc values are contiguous in the IP field (row
+      // r is point r mod q of set tet r / q)
+      for (int r = gl; r < mrows; r += LANES) {
+        const int s = q == 4 ? r >> 2 : r / q, l = q == 4 ? r & 3 : r % q;
+        const double* src = a.ip + ((int64_t)sS[g][s] * q + l) * nc;
+        for (int c = 0; c < nc; ++c) sV[g][r * nc + c] = src[c];
       }
     __syncthreads();
     // y = transpose(Q) b, sequential over rows (mth/mth_def.h:221-232)
     if (ok)
-      for (int w = gl; w < NT * nc; w += LANES) {
-        int i = w % NT, c = w / NT;
-        sY[g][c * NT + i] = seq_sum(0, mrows, [&](int r) {
+      // two (component, coefficient) sums per lane, w and w + LANES
+      for (int w = gl; w < NT * nc; w += 2 * LANES) {
+        const bool two = w + LANES < NT * nc;
+        const int i0 = w % NT, c0 = w / NT;
+        const int w1 = two ? w + LANES : w;
+        const int i1 = w1 % NT, c1 = w1 / NT;
+        auto prod = [&](int i, int c, int r) {
           const double b = staged ? sV[g][r * nc + c] : a.ip[((int64_t)sS[g][r / q] * q + r % q) * nc + c];
           return sR[g][r * NT + i] * b;
-        });
+        };
+        double y0, y1;
+        seq_sum2(0, mrows, [&](int r) { return prod(i0, c0, r); }, [&](int r) { return prod(i1, c1, r); }, y0, y1);
+        sY[g][c0 * NT + i0] = y0;
+        if (two) sY[g][c1 * NT + i1] = y1;
       }
     __syncthreads();
     // backsubUT (mth/mthQR.cc:131-148) and evalPolynomial at the node
@@ -447,7 +484,7 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
   __shared__ double sR[EXACT ? XMAXR * NT : 1];
   __shared__ double sV[EXACT ? NT * XMAXR : 1];
   __shared__ double sDot[NT];
-  __shared__ double sNorm[2];
+  __shared__ double sNorm[3];
   // the candidate hash during enumeration, the screen's sample points after
   // it (barriers separate the two uses)
   __shared__ double sD[3 * XMAXR];
@@ -696,7 +733,7 @@ __global__ __launch_bounds__(64) void k_exact_big(FitArgs a, double* ws)
   __shared__ double sRt[NT * NT];
   __shared__ double sY[27 * NT];
   __shared__ double sDot[NT];
-  __shared__ double sNorm[2];
+  __shared__ double sNorm[3];
   __shared__ int32_t sS[BIG_MMAX];
   const int lane = threadIdx.x;
   const MeshView& m = a.m;
