@@ -234,13 +234,14 @@ __global__ __launch_bounds__(256) void k_classify(FitArgs a, int nterms)
 enum { FIT_REGULAR = 0, FIT_EXPANDED = 1, FIT_MIXED = 2 };
 
 // Waves per SIMD the fit of a bucket is compiled for: the kernel is bound by
-// the latency of its sequential sums, so the small buckets (most patches)
-// ask for more resident waves; the Q row of MMAX doubles per lane sets the
-// floor of the register budget.
+// the latency of its sequential sums, so the buckets that hold most patches
+// ask for two resident waves (the LDS of the 16- and 32-row buckets allows
+// no more); the Q row of MMAX doubles per lane sets the floor of the
+// register budget.
 // (Only targets that compile without scratch spills are requested.)
 constexpr int exact_waves(int nt, int mmax)
 {
-  return nt == 4 ? (mmax == 32 ? 2 : 1) : (mmax <= 32 ? 4 : mmax == 96 ? 2 : 1);
+  return nt == 4 ? (mmax == 32 ? 2 : 1) : (mmax <= 32 || mmax == 96 ? 2 : 1);
 }
 
 template <int NT, int MMAX, int LANES, int MODE>

@@ -34,11 +34,11 @@ struct DBuf {
   int ensure(size_t n)
   {
     if (n <= bytes) return 0;
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
     if (hipMalloc(&p, n) != hipSuccess) {
-      hipGetLastError();
+      (void)hipGetLastError();
       return sprg_fail(SPRG_ENOMEM, "device allocation of %zu bytes failed", n);
     }
     bytes = n;
@@ -46,7 +46,7 @@ struct DBuf {
   }
   void release()
   {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -79,7 +79,7 @@ struct sprg_mesh {
   int phase_last = 0;  // highest phase recorded by the last call
   void mark(int i)
   {
-    if (ev_ok) hipEventRecord(ev[i], ctx->stream);
+    if (ev_ok) (void)hipEventRecord(ev[i], ctx->stream);
     if (i > phase_last) phase_last = i;
   }
 
@@ -174,7 +174,7 @@ extern "C" int sprg_ctx_create(int device, sprg_ctx** out)
     return sprg_fail(SPRG_EHIP, "ctx_create: stream creation failed");
   }
   if (int rc = scratch_address(device, &c->scratch)) {
-    hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return rc;
   }
@@ -185,9 +185,9 @@ extern "C" int sprg_ctx_create(int device, sprg_ctx** out)
 extern "C" int sprg_ctx_destroy(sprg_ctx* ctx)
 {
   if (!ctx) return 0;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  hipStreamDestroy(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;
 }
@@ -206,7 +206,7 @@ extern "C" int sprg_malloc(sprg_ctx* ctx, size_t bytes, void** dptr)
   if (!ctx || !dptr) return sprg_fail(SPRG_EINVAL, "malloc: null argument");
   HIP_OK(hipSetDevice(ctx->device), "hipSetDevice");
   if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess) {
-    hipGetLastError();
+    (void)hipGetLastError();
     *dptr = nullptr;
     return sprg_fail(SPRG_ENOMEM, "malloc: %zu bytes", bytes);
   }
@@ -498,8 +498,8 @@ extern "C" int sprg_mesh_create(sprg_ctx* ctx, int64_t nv, int64_t nt, int64_t n
 extern "C" int sprg_mesh_destroy(sprg_mesh* m)
 {
   if (!m) return 0;
-  hipSetDevice(m->ctx->device);
-  hipStreamSynchronize(m->ctx->stream);
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
   DBuf* all[] = {&m->tv, &m->te, &m->xyz, &m->v2t_off, &m->v2t, &m->e2t_off, &m->e2t,
                  &m->lists, &m->xset, &m->rec, &m->elem, &m->partial, &m->scalars,
                  &m->status, &m->setlen, &m->big_ws, &m->t_slot_off, &m->t_slot, &m->t_tet_off,
@@ -1193,8 +1193,8 @@ extern "C" int sprg_batch_create(sprg_ctx* ctx, int nmesh, sprg_mesh* const* mes
 extern "C" int sprg_batch_destroy(sprg_batch* b)
 {
   if (!b) return 0;
-  hipSetDevice(b->ctx->device);
-  hipStreamSynchronize(b->ctx->stream);
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
   b->drop();
   b->bstat.release();
   delete b;
