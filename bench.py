@@ -32,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 
 
 def log(msg):
@@ -47,6 +48,17 @@ def algorithmic_bytes(T, V, E, q=4, c=9, order=2):
     size = T * q * c * 8 + N * c * 8 + T * k * 4 + N * 24 + T * 8
     average = T * 8 + 4 * T * 4 + (V + 1) * 4 + V * 8
     return {"recover": recover, "size": size, "average": average}
+
+
+def fit_flops(T, V, E, q=4, order=2):
+    """Algorithmic FP64 flops of the fast form's regular fit: every patch
+    fits q rows per tet of its set (each tet is in 4 vertex and, on a P2
+    mesh, 6 edge patches); per row 74 flops for the Gram moments (3 sub,
+    6 mul, 3 add, 31 FMA) and 56 for the weighted sum (3 sub, 6 mul, 10 FMA,
+    9 sub, 9 FMA of 9 components); per patch about 900 for the 10x10
+    Cholesky factor and its two triangular solves."""
+    rows = T * q * (4 + (6 if order == 2 else 0))
+    return 130 * rows + 900 * (V + (E if order == 2 else 0))
 
 
 def copy_probe(torch, nbytes=1 << 30, reps=5):
@@ -247,6 +259,13 @@ def main():
         "phases_ms": {"fit": phases[0], "expand": phases[1], "expanded_fit": phases[2],
                       "element": phases[3], "factor_average": phases[4]},
         "routes": routes,
+        "roofline_fp64": ({"bound": "fp64 vector ALU", "achieved": fit_flops(T, V, E) / (fit_ms / 1e3) / 1e12,
+                           "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": fit_flops(T, V, E) / (fit_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
+                           "flops": fit_flops(T, V, E),
+                           "note": "the same launch priced by its algorithmic FP64 flops, the bound it "
+                                   "is held by (its HBM traffic moves at a fraction of peak)"}
+                          if args.form == "fp64f" else None),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(args.form),
                      "measured_copy_gbs": probe["gbs"],
