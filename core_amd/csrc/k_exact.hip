@@ -343,12 +343,18 @@ __global__ __launch_bounds__(64, exact_waves(NT, MMAX)) void k_exact_fit(FitArgs
         const int i0 = w % NT, c0 = w / NT;
         const int w1 = two ? w + LANES : w;
         const int i1 = w1 % NT, c1 = w1 / NT;
-        auto prod = [&](int i, int c, int r) {
-          const double b = staged ? sV[g][r * nc + c] : a.ip[((int64_t)sS[g][r / q] * q + r % q) * nc + c];
-          return sR[g][r * NT + i] * b;
-        };
+        // b from LDS or from the IP field, the branch outside the sums (a
+        // select of the two addresses would make every load a generic one)
         double y0, y1;
-        seq_sum2(0, mrows, [&](int r) { return prod(i0, c0, r); }, [&](int r) { return prod(i1, c1, r); }, y0, y1);
+        if (staged) {
+          auto prod = [&](int i, int c, int r) { return sR[g][r * NT + i] * sV[g][r * nc + c]; };
+          seq_sum2(0, mrows, [&](int r) { return prod(i0, c0, r); }, [&](int r) { return prod(i1, c1, r); }, y0, y1);
+        } else {
+          auto prod = [&](int i, int c, int r) {
+            return sR[g][r * NT + i] * a.ip[((int64_t)sS[g][r / q] * q + r % q) * nc + c];
+          };
+          seq_sum2(0, mrows, [&](int r) { return prod(i0, c0, r); }, [&](int r) { return prod(i1, c1, r); }, y0, y1);
+        }
         sY[g][c0 * NT + i0] = y0;
         if (two) sY[g][c1 * NT + i1] = y1;
       }
@@ -597,7 +603,7 @@ __global__ __launch_bounds__(64) void k_expand(FitArgs a)
               // past 256 distinct tets the set is too large anyway;
               // stopping here keeps the hash far from full so every probe
               // ends
-              if (*(volatile int*)&sN >= 256) break;
+              if (__atomic_load_n(&sN, __ATOMIC_RELAXED) >= 256) break;
               uint32_t h = ((uint32_t)u[j] * 2654435761u) & (HSIZE - 1);
               for (;;) {
                 int32_t prev = atomicCAS(&sHash[h], -1, u[j]);
