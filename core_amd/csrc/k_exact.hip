@@ -174,9 +174,21 @@ __device__ __forceinline__ bool r_recurrence(double* R, double* V, double* sDot,
       sDot[gl] = seq_sum(k, m, [&](int i) { return R[i * NT + gl] * v[i]; });
     }
     __syncthreads();
-    if (ok)
-      for (int i = k + gl; i < m; i += LANES)
-        for (int j = k; j < NT; ++j) R[i * NT + j] -= 2 * sDot[j] * V[k * MS + i];
+    if (ok) {
+      // reflect_columns' update a(i,j) -= 2 dot_j v(i) for j >= k, with the
+      // columns at compile-time positions (the loads of a row in flight
+      // together) and 2 dot_j read once per reflector
+      double d2[NT];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) d2[j] = 2 * sDot[j];
+      for (int i = k + gl; i < m; i += LANES) {
+        const double vi = V[k * MS + i];
+        double* Ri = R + i * NT;
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          if (j >= k) Ri[j] -= d2[j] * vi;
+      }
+    }
     __syncthreads();
   }
   return ok;
