@@ -214,7 +214,22 @@ __global__ __launch_bounds__(EST_T) void k_factor(EstArgs a)
 {
   __shared__ double red[2][EST_T];
   double s0 = 0, s1 = 0;
-  for (int i = threadIdx.x; i < a.nblocks; i += EST_T) {
+  // each thread's partials in order, eight loads in flight at a time
+  int i = threadIdx.x;
+  for (; i + 7 * EST_T < a.nblocks; i += 8 * EST_T) {
+    double p[16];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      p[2 * u] = a.partial[2 * (i + u * EST_T)];
+      p[2 * u + 1] = a.partial[2 * (i + u * EST_T) + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      s0 += p[2 * u];
+      s1 += p[2 * u + 1];
+    }
+  }
+  for (; i < a.nblocks; i += EST_T) {
     s0 += a.partial[2 * i];
     s1 += a.partial[2 * i + 1];
   }
@@ -264,10 +279,20 @@ __global__ void k_average(EstArgs a)
     if (h_new > a.beta * h) h_new = a.beta * h;
     return h_new;
   };
-  // tets four at a time so their loads overlap; the sum keeps the
-  // adjacency order
+  // tets eight, then four, at a time so their loads overlap; the sum keeps
+  // the adjacency order
   double s = 0;
   int32_t k = b;
+  for (; k + 8 <= e; k += 8) {
+    int32_t t[8];
+    double es[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t[j] = m.v2t[k + j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) es[j] = elem_size(t[j]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += es[j];
+  }
   for (; k + 4 <= e; k += 4) {
     int32_t t[4];
     double es[4];
