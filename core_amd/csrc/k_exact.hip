@@ -415,13 +415,11 @@ __device__ __forceinline__ int shared_verts(const int32_t* a, const int32_t* b)
   return s;
 }
 
-// Monomial term k of poly_terms as v[a] * v[b] over v = (1, x, y, z): the
-// same products poly_terms forms (a unit factor is exact).
-__device__ __forceinline__ double term_of(int a, int b, const double* d)
+__device__ __forceinline__ double readlane_d(double v, int lane)
 {
-  const double va = a == 0 ? 1.0 : a == 1 ? d[0] : a == 2 ? d[1] : d[2];
-  const double vb = b == 0 ? 1.0 : b == 1 ? d[0] : b == 2 ? d[1] : d[2];
-  return va * vb;
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
 }
 
 __device__ __forceinline__ void term_factors(int k, int* a, int* b)
@@ -467,19 +465,43 @@ __device__ bool certify_full_rank(const MeshView& m, int q, const int32_t* S, in
   int ra, rb, ca, cb;
   term_factors(r, &ra, &rb);
   term_factors(c, &ca, &cb);
+  // this lane's two monomials as v[a] * v[b] over v = (1, x, y, z) (the
+  // products poly_terms forms; a unit factor is exact), each factor a
+  // one-hot combination of v (the zero weights add +-0, so a factor is
+  // exactly its component), the rows four at a time
+  const double wa[4] = {ra == 0 ? 1.0 : 0.0, ra == 1 ? 1.0 : 0.0, ra == 2 ? 1.0 : 0.0, ra == 3 ? 1.0 : 0.0};
+  const double wb[4] = {rb == 0 ? 1.0 : 0.0, rb == 1 ? 1.0 : 0.0, rb == 2 ? 1.0 : 0.0, rb == 3 ? 1.0 : 0.0};
+  const double wc[4] = {ca == 0 ? 1.0 : 0.0, ca == 1 ? 1.0 : 0.0, ca == 2 ? 1.0 : 0.0, ca == 3 ? 1.0 : 0.0};
+  const double wd[4] = {cb == 0 ? 1.0 : 0.0, cb == 1 ? 1.0 : 0.0, cb == 2 ? 1.0 : 0.0, cb == 3 ? 1.0 : 0.0};
+  auto pick = [](const double* w, const double* d) { return fma(w[3], d[2], fma(w[2], d[1], fma(w[1], d[0], w[0]))); };
   double g = 0;
-  if (own)
-    for (int i = 0; i < nrows; ++i) {
-      const double d[3] = {sD[3 * i], sD[3 * i + 1], sD[3 * i + 2]};
-      g = fma(term_of(ra, rb, d), term_of(ca, cb, d), g);
+  if (own) {
+    int i = 0;
+    for (; i + 4 <= nrows; i += 4) {
+      double ta[4], tb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double d[3] = {sD[3 * (i + u)], sD[3 * (i + u) + 1], sD[3 * (i + u) + 2]};
+        ta[u] = pick(wa, d) * pick(wb, d);
+        tb[u] = pick(wc, d) * pick(wd, d);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) g = fma(ta[u], tb[u], g);
     }
+    for (; i < nrows; ++i) {
+      const double d[3] = {sD[3 * i], sD[3 * i + 1], sD[3 * i + 2]};
+      g = fma(pick(wa, d) * pick(wb, d), pick(wc, d) * pick(wd, d), g);
+    }
+  }
   __syncthreads();  // sD is free again
   const double diag_own = g;
   bool rank_ok = true;
   for (int k = 0; k < NT; ++k) {
     const int kk = k * (k + 1) / 2 + k;
-    const double dk = __shfl(g, kk);
-    const double diag = __shfl(diag_own, kk);
+    // the pivot's lane is the same for the whole wave: a lane read, not a
+    // shuffle through LDS
+    const double dk = readlane_d(g, kk);
+    const double diag = readlane_d(diag_own, kk);
     const double lkk = dk > 0 ? sqrt(dk) : 0.0;
     if (!(diag > 0 && dk >= 1e-6 * diag) || !(lkk >= 1.05e-10)) rank_ok = false;
     const double piv = lkk > 0 ? lkk : 1.0;

@@ -85,6 +85,8 @@ inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask,
   const int from = (l & ~3) | ((ctrl >> (2 * (l & 3))) & 3);
   return (int)emu::shfl64(src, from);
 }
+// the value of lane `lane` of the wave (every lane calls it)
+inline int __builtin_amdgcn_readlane(int src, int lane) { return (int)emu::shfl64(src, lane); }
 inline int __builtin_amdgcn_mov_dpp(int src, int ctrl, int row_mask, int bank_mask, bool bound)
 {
   return __builtin_amdgcn_update_dpp(0, src, ctrl, row_mask, bank_mask, bound);
