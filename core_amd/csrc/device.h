@@ -159,7 +159,7 @@ __device__ __forceinline__ void wave_append(int target, int my_target, int32_t* 
   const int leader = __ffsll((unsigned long long)mask) - 1;
   int base = 0;
   if (lane == leader) base = atomicAdd(count, __popcll(mask));
-  base = __shfl(base, leader);
+  base = __builtin_amdgcn_readlane(base, leader);  // leader: the same lane for the whole wave
   if (my_target == target) {
     const uint64_t below = mask & ((1ull << lane) - 1);
     list[base + __popcll(below)] = v;
