@@ -50,20 +50,68 @@ __device__ __forceinline__ int64_t node_id(const MeshView& m, int64_t t, int n)
 //     x_{i+1} - x_0 exactly, so J is formed once.
 //   - The coordinates are used first (element size, Jacobians) and are dead
 //     before the recovered values accumulate.
+//   - A tet's Q x NC IP values are contiguous, so a lane that loads its own
+//     tet's values touches a different cache line in every lane of every
+//     load.  When the nodes go outer, the block loads its tets' IP values
+//     instead as one contiguous run (16 bytes per lane, lanes in address
+//     order), TB tets per pass, into LDS, and each lane reads its own tet's
+//     values there.  A tet's values sit at an odd stride of RP doubles, so
+//     the lanes of one read fall in different banks.  Only the source of the
+//     values changes: the sums and their order are the same.
+constexpr int elem_points(int ipo) { return ipo <= 1 ? 1 : ipo == 2 ? 4 : 5; }
+// nodes outer needs Q x NC accumulators; above 36 the points go outer
+constexpr bool elem_nodes_outer(int ipo, int nc) { return elem_points(ipo) * nc <= 36; }
+// three blocks per CU (three waves per SIMD) when the nodes go outer: 168
+// VGPRs, and the staging buffers below leave 3 x 52 KB of LDS
+constexpr int elem_waves(int ipo, int nc) { return elem_nodes_outer(ipo, nc) ? 3 : 1; }
+
 template <int ORDER, int NC, int IPO>
-__global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
+__global__ __launch_bounds__(EST_T, elem_waves(IPO, NC)) void k_element(EstArgs a)
 {
   constexpr int NN = ORDER == 1 ? 4 : 10;
-  constexpr int Q = IPO <= 1 ? 1 : IPO == 2 ? 4 : 5;
-  // nodes outer needs Q x NC accumulators; above 36 the points go outer
-  constexpr bool NODES_OUTER = Q * NC <= 36;
+  constexpr int Q = elem_points(IPO);
+  constexpr bool NODES_OUTER = elem_nodes_outer(IPO, NC);
+  constexpr int R = Q * NC;  // IP values per tet
+  constexpr int RP = R | 1;  // their LDS stride
+  constexpr int STAGE_BYTES = 40960;
+  constexpr int TB = EST_T * RP * 8 <= STAGE_BYTES         ? EST_T
+                     : (EST_T / 2) * RP * 8 <= STAGE_BYTES ? EST_T / 2
+                                                           : EST_T / 4;
+  static_assert(!NODES_OUTER || TB * RP * 8 <= STAGE_BYTES, "staging pass too large");
+  constexpr int NCHUNK = TB * R / 2;                      // 16-byte chunks per pass
+  constexpr int CHUNKS = (NCHUNK + EST_T - 1) / EST_T;  // per thread
   __shared__ double red[2][EST_T];
+  __shared__ double stage[NODES_OUTER ? TB * RP : 1];
+  __shared__ double hold[Q + 1][EST_T];  // a thread's dV per point and h
   const MeshView& m = a.m;
   // logical block: its tets and its slot in the fixed-order partial sums
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x, XCD_K_ELEM);
-  const int64_t t = blk * EST_T + threadIdx.x;
+  const int64_t t0 = blk * EST_T;
+  const int64_t t = t0 + threadIdx.x;
+  const bool live = t < m.nt;
   double self = 0, glob = 0;
-  if (t < m.nt) {
+  double h = 0;      // getCurrentSize
+  double sum = 0;    // ElementError::sum
+  double dV[Q];
+  double v2[NODES_OUTER ? Q : 1][NC];
+  auto point = [&](int p, const double* v2p, const double* v1) {
+    const double w = rule_weight(IPO, p);
+    // SelfProduct::atPoint: r += (v*v)*w*dV
+    double vv = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) vv += v2p[c] * v2p[c];
+    const double dVp = hold[p][threadIdx.x];
+    self += vv * w * dVp;
+    // ElementError::atPoint: diff = v1 - v2; sum += (diff*diff)*w*dV
+    double dd = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const double d = v1[c] - v2p[c];
+      dd += d * d;
+    }
+    sum += dd * w * dVp;
+  };
+  if (live) {
     int64_t nid[NN];
     {
       const int4 vv = *reinterpret_cast<const int4*>(m.tv + 4 * t);
@@ -78,7 +126,6 @@ __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
 #pragma unroll
       for (int j = 0; j < 3; ++j) X[n][j] = m.xyz[3 * nid[n] + j];
     // getCurrentSize: max edge measure (spr/sprEstimateError.cc:219-228)
-    double h = 0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       const double* x0 = X[tet_edge_vert(k, 0)];
@@ -87,7 +134,7 @@ __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
       const double e = edge_measure(ORDER, x0, x1, xm);
       h = (h < e) ? e : h;
     }
-    double dV[Q];
+    hold[Q][threadIdx.x] = h;  // held in LDS until the end: registers are short
     if constexpr (ORDER == 1) {
       double J[9];
 #pragma unroll
@@ -117,28 +164,11 @@ __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
         dV[p] = det3(J);
       }
     }
-    double sum = 0;  // ElementError::sum
-    auto point = [&](int p, const double* v2) {
-      const double w = rule_weight(IPO, p);
-      // SelfProduct::atPoint: r += (v*v)*w*dV
-      double vv = 0;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) vv += v2[c] * v2[c];
-      self += vv * w * dV[p];
-      // ElementError::atPoint: diff = v1 - v2; sum += (diff*diff)*w*dV
-      const double* v1 = a.ip + (t * Q + p) * NC;
-      double dd = 0;
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const double d = v1[c] - v2[c];
-        dd += d * d;
-      }
-      sum += dd * w * dV[p];
-    };
+    for (int p = 0; p < Q; ++p) hold[p][threadIdx.x] = dV[p];
     // the coordinates are dead from here; keep the value loads below it
     asm volatile("" ::: "memory");
     if constexpr (NODES_OUTER) {
-      double v2[Q][NC];
 #pragma unroll
       for (int n = 0; n < NN; ++n) {
         double r[NC];
@@ -158,12 +188,10 @@ __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
           }
         }
       }
-#pragma unroll
-      for (int p = 0; p < Q; ++p) point(p, v2[p]);
     } else {
 #pragma unroll 1
       for (int p = 0; p < Q; ++p) {
-        double xi[3], N[NN], v2[NC];
+        double xi[3], N[NN], v2p[NC];
         rule_point(IPO, p, xi);
         tet_values(ORDER, xi, N);
 #pragma unroll 1
@@ -172,12 +200,61 @@ __global__ __launch_bounds__(EST_T) void k_element(EstArgs a)
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             const double v = r[c] * N[n];
-            v2[c] = n == 0 ? 0 + v : v2[c] + v;
+            v2p[c] = n == 0 ? 0 + v : v2p[c] + v;
           }
         }
-        point(p, v2);
+        point(p, v2p, a.ip + (t * Q + p) * NC);
       }
     }
+  }
+  if constexpr (NODES_OUTER) {
+    // the staging loads start after the gathers above
+    asm volatile("" ::: "memory");
+#pragma unroll 1
+    for (int pass = 0; pass < EST_T / TB; ++pass) {
+      const int64_t base = t0 + (int64_t)pass * TB;  // first tet of the pass
+      const int64_t left = m.nt - base;
+      const int nval = left <= 0 ? 0 : left >= TB ? TB * R : (int)left * R;
+      const double* src = a.ip + base * R;
+      if (pass > 0) __syncthreads();  // the previous pass's reads are done
+      // LDS slot of the pass's IP value d: tet d / R at stride RP
+      auto slot = [](int d) { return d + (RP - R) * (d / R); };
+      double x[CHUNKS][2];
+      if (nval == TB * R) {  // a full pass: every block but the mesh's last
+#pragma unroll
+        for (int i = 0; i < CHUNKS; ++i) {
+          const int d = 2 * (threadIdx.x + i * EST_T);
+          if (CHUNKS * EST_T == NCHUNK || d < 2 * NCHUNK) {
+            x[i][0] = src[d];
+            x[i][1] = src[d + 1];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < CHUNKS; ++i) {
+          const int d = 2 * (threadIdx.x + i * EST_T);
+          x[i][0] = d < nval ? src[d] : 0;
+          x[i][1] = d + 1 < nval ? src[d + 1] : 0;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < CHUNKS; ++i) {
+        const int d = 2 * (threadIdx.x + i * EST_T);
+        if (CHUNKS * EST_T == NCHUNK || d < 2 * NCHUNK) {
+          stage[slot(d)] = x[i][0];
+          stage[slot(d + 1)] = x[i][1];
+        }
+      }
+      __syncthreads();
+      const int lt = (int)threadIdx.x - pass * TB;
+      if (live && lt >= 0 && lt < TB) {
+#pragma unroll
+        for (int p = 0; p < Q; ++p) point(p, v2[p], stage + lt * RP + p * NC);
+      }
+    }
+  }
+  if (live) {
+    h = hold[Q][threadIdx.x];
     const double d = 3, po = ORDER;
     glob = pow(sqrt(sum), ((2 * d) / (2 * po + d)));
     if (!a.target) {
