@@ -1281,6 +1281,7 @@ int workspace_key(sprg_batch* b, int q, int nc, int form, std::vector<uint64_t>&
     key.push_back((uint64_t)m->shape.fast_blocks);
     key.push_back((uint64_t)m->shape.exact_blocks);
     key.push_back((uint64_t)m->shape.expand_blocks);
+    key.push_back((uint64_t)m->shape.tile_blocks);
   }
   return 0;
 }
