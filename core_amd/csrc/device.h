@@ -111,6 +111,8 @@ struct TileView {
   const int32_t* leftover = nullptr;  // patches outside every tile
   int64_t nleftover = 0;
 };
+constexpr int TILE_DESC = 6;         // ints describing a tile to its build
+constexpr int TILE_SORT_CAP = 2048;  // candidate tets a tile build sorts
 constexpr int TILE_B = 64;           // lanes per tile: one wave, so a tile's
                                      // barriers never wait on other waves
 // tets a tile union may hold: two per lane (the staging and its prefetch
@@ -183,14 +185,15 @@ int launch_exact_big(const FitArgs& a, int nterms, double* ws, int grid, hipStre
 int launch_fast_fit(const FitArgs& a, int nterms, int form, int src, hipStream_t s);
 // The regular pass of the fast forms over the mesh's tiles (k_tile.hip).
 int launch_tile_fit(const FitArgs& a, int nterms, int form, hipStream_t s);
-// Tile construction: tile_vert [2 ntile] (the vertex run [begin, end) of
-// each tile), own_off/own_e (edges owned by each vertex, CSR; null for an
-// order-1 mesh); fills tet_off (device, [ntile+1]), tets and the loc arrays.
-// pass 0 writes each tile's union size to tet_off[tile + 1], pass 1 (after
-// the scan) the unions and positions.
-int launch_tile_build(const MeshView& m, int64_t ntile, const int32_t* tile_vert,
-                      const int32_t* own_off, const int32_t* own_e, int pass, int32_t* tet_off,
-                      int32_t* tets, uint16_t* v2t_loc, uint16_t* e2t_loc, int32_t* status,
+// Tile construction: desc [TILE_DESC ntile] (each tile's candidate vertex
+// rows [cv0, cv1), the vertices whose patches it fits [pv0, pv1) and the
+// owned edges it fits own_e[pe0, pe1)), own_e (the edges grouped by their
+// owning vertex, in vertex order; null for an order-1 mesh); fills tet_off
+// (device, [ntile+1]), tets and the loc arrays.  pass 0 writes each tile's
+// union size to tet_off[tile + 1], pass 1 (after the scan) the unions and
+// the positions of the tile's patches' tets.
+int launch_tile_build(const MeshView& m, int64_t ntile, const int32_t* desc, const int32_t* own_e, int pass,
+                      int32_t* tet_off, int32_t* tets, uint16_t* v2t_loc, uint16_t* e2t_loc, int32_t* status,
                       hipStream_t s);
 int launch_grad(const MeshView& m, int field_order, int nc_in,
                 const double* nodal, int ip_order, double* ip, hipStream_t s);

@@ -37,35 +37,33 @@
 namespace sprg {
 namespace {
 
-constexpr int SORT_CAP = 2048; // candidate tets a tile build sorts (sum of valences)
 
 // ---------------------------------------------------------------------------
 // tile construction (once per mesh)
 
 // One block per tile.  The v2t rows of consecutive vertices are contiguous
-// in the CSR, so the candidate list of the tile's vertex run [v0, v1)
-// (tile_vert[2 tile], tile_vert[2 tile + 1]) is v2t[v2t_off[v0] ..
-// v2t_off[v1]).  It
-// is sorted in LDS (bitonic network padded with INT32_MAX) and compacted to
-// the union.  pass 0 writes the union size to tet_off[tile + 1]; pass 1
-// writes the union at tet_off[tile] and, for every patch of the tile, the
-// position of each of its tets in the union (status bit 1: a tet of an
-// owned edge is missing from the union, i.e. tet->edge and tet->vertex
-// disagree).
-__global__ __launch_bounds__(256) void k_tile_union(MeshView m, const int32_t* tile_vert,
-                                                    const int32_t* own_off, const int32_t* own_e,
+// in the CSR, so the candidate list of the tile's rows [cv0, cv1)
+// (desc[TILE_DESC tile ..]: cv0, cv1, pv0, pv1, pe0, pe1) is
+// v2t[v2t_off[cv0] .. v2t_off[cv1]).  It is sorted in LDS (bitonic network
+// padded with INT32_MAX) and compacted to the union.  pass 0 writes the
+// union size to tet_off[tile + 1]; pass 1 writes the union at tet_off[tile]
+// and, for the tile's patches -- the vertices [pv0, pv1) and the edges
+// own_e[pe0, pe1), whose owners' rows are candidates -- the position of
+// each of their tets in the union (status bit 1: a tet of an owned edge is
+// missing from the union, i.e. tet->edge and tet->vertex disagree).
+__global__ __launch_bounds__(256) void k_tile_union(MeshView m, const int32_t* desc, const int32_t* own_e,
                                                     int pass, int32_t* tet_off, int32_t* tets,
                                                     uint16_t* v2t_loc, uint16_t* e2t_loc,
                                                     int32_t* status)
 {
-  __shared__ int32_t sA[SORT_CAP];
+  __shared__ int32_t sA[TILE_SORT_CAP];
   __shared__ int32_t sScan[256];
-  __shared__ int32_t sU[SORT_CAP];
+  __shared__ int32_t sU[TILE_SORT_CAP];
   const int tid = threadIdx.x;
   const int64_t tile = blockIdx.x;
-  const int32_t v0 = tile_vert[2 * tile], v1 = tile_vert[2 * tile + 1];
-  const int32_t k0 = m.v2t_off[v0], k1 = m.v2t_off[v1];
-  const int n = k1 - k0;  // <= SORT_CAP (the host caps the sum of valences)
+  const int32_t* const d = desc + TILE_DESC * tile;
+  const int32_t k0 = m.v2t_off[d[0]], k1 = m.v2t_off[d[1]];
+  const int n = k1 - k0;  // <= TILE_SORT_CAP (the host caps the sum of valences)
   int P = 1;
   while (P < n) P <<= 1;
   for (int i = tid; i < P; i += 256) sA[i] = i < n ? m.v2t[k0 + i] : INT32_MAX;
@@ -86,7 +84,7 @@ __global__ __launch_bounds__(256) void k_tile_union(MeshView m, const int32_t* t
       __syncthreads();
     }
   // first occurrences, scanned in chunks of 8 per thread
-  constexpr int PER = SORT_CAP / 256;
+  constexpr int PER = TILE_SORT_CAP / 256;
   int flags[PER];
   int cnt = 0;
 #pragma unroll
@@ -123,10 +121,9 @@ __global__ __launch_bounds__(256) void k_tile_union(MeshView m, const int32_t* t
     }
     return (total > 0 && sU[lo] == t) ? lo : -1;
   };
-  for (int k = k0 + tid; k < k1; k += 256) v2t_loc[k] = (uint16_t)find(m.v2t[k]);
-  if (!own_off) return;
-  const int32_t e0 = own_off[v0], e1 = own_off[v1];
-  for (int i = e0 + tid; i < e1; i += 256) {
+  for (int k = m.v2t_off[d[2]] + tid; k < m.v2t_off[d[3]]; k += 256) v2t_loc[k] = (uint16_t)find(m.v2t[k]);
+  if (!own_e) return;
+  for (int i = d[4] + tid; i < d[5]; i += 256) {
     const int32_t e = own_e[i];
     for (int32_t k = m.e2t_off[e]; k < m.e2t_off[e + 1]; ++k) {
       const int u = find(m.e2t[k]);
@@ -645,14 +642,12 @@ int launch_tile_p2(const FitArgs& a, float tau_rank, float tau_acc, hipStream_t 
 
 }  // namespace
 
-int launch_tile_build(const MeshView& m, int64_t ntile, const int32_t* tile_vert,
-                      const int32_t* own_off, const int32_t* own_e, int pass, int32_t* tet_off,
-                      int32_t* tets, uint16_t* v2t_loc, uint16_t* e2t_loc, int32_t* status,
+int launch_tile_build(const MeshView& m, int64_t ntile, const int32_t* desc, const int32_t* own_e, int pass,
+                      int32_t* tet_off, int32_t* tets, uint16_t* v2t_loc, uint16_t* e2t_loc, int32_t* status,
                       hipStream_t s)
 {
   if (ntile > 0)
-    k_tile_union<<<(unsigned)ntile, 256, 0, s>>>(m, tile_vert, own_off, own_e, pass, tet_off, tets,
-                                                  v2t_loc, e2t_loc, status);
+    k_tile_union<<<(unsigned)ntile, 256, 0, s>>>(m, desc, own_e, pass, tet_off, tets, v2t_loc, e2t_loc, status);
   return hipGetLastError() == hipSuccess ? 0 : sprg_fail(SPRG_EHIP, "tile build launch failed");
 }
 

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <iterator>
 #include <mutex>
 #include <new>
 #include <string>
@@ -246,14 +247,15 @@ namespace {
 int group_log2(int64_t ns) { return ns <= 6 ? 0 : ns <= 12 ? 1 : ns <= 24 ? 2 : 3; }
 
 // Builds the tiles of the regular fast pass (k_tile.hip) from the host
-// connectivity and the device CSR: the host groups consecutive vertices
-// (with the edges they own, the lower vertex owning an edge) into tiles of
-// at most TILE_B lanes whose candidate lists (sum of vertex valences) fit
-// the build's sort; the device sorts each tile's union and records every
-// patch's positions in it.  Tiles whose union exceeds TILE_MAX_UNION are
-// rebuilt with a smaller candidate cap.  A vertex too large to share a tile
-// with anything (more than TILE_B lanes, or a valence above the cap) stays
-// outside, with its owned edges: the thread-per-patch pass fits those.
+// connectivity and the device CSR: the host takes the patches in vertex
+// order (each vertex's patch, then the edges it owns, the lower vertex
+// owning an edge) into tiles of at most TILE_B lanes whose tet unions (of
+// the v2t rows of the vertices whose patches or edges they take) hold at
+// most TILE_MAX_UNION tets -- a tile may end inside a vertex's edges, so
+// tiles fill their lanes -- and the device sorts each tile's union and
+// records every patch's positions in it.  A vertex with more than
+// TILE_MAX_UNION tets stays outside, with its owned edges: the
+// thread-per-patch pass fits those.
 int build_tiles(sprg_mesh* m, const int32_t* tet_verts, const int32_t* tet_edges)
 {
   const int64_t nv = m->nv, nt = m->nt;
@@ -283,69 +285,111 @@ int build_tiles(sprg_mesh* m, const int32_t* tet_verts, const int32_t* tet_edges
     std::vector<int32_t> cur(own_off.begin(), own_off.end() - 1);
     for (int64_t e = 0; e < ne; ++e) own_e[cur[owner[e]]++] = (int32_t)e;
   }
-  std::vector<int32_t> vslots(nv);
-  for (int64_t v = 0; v < nv; ++v) {
-    int sl = 1 << group_log2(vcount[v]);
-    for (int32_t i = own_off[v]; i < own_off[v + 1]; ++i) sl += 1 << group_log2(ecount[own_e[i]]);
-    vslots[v] = sl;
-  }
   hipStream_t s = m->ctx->stream;
-  DBuf d_vert, d_own_off, d_own_e, d_status;
+  DBuf d_desc, d_own_e, d_status;
   int rc = 0;
-  if ((rc = d_own_off.ensure(sizeof(int32_t) * (nv + 1))) || (rc = d_own_e.ensure(sizeof(int32_t) * (ne + 1))) ||
-      (rc = d_status.ensure(sizeof(int32_t))))
-    return rc;
-  HIP_OK(hipMemcpyAsync(d_own_off.p, own_off.data(), sizeof(int32_t) * (nv + 1), hipMemcpyHostToDevice, s), "tiles");
+  if ((rc = d_own_e.ensure(sizeof(int32_t) * (ne + 1))) || (rc = d_status.ensure(sizeof(int32_t)))) return rc;
   if (ne) HIP_OK(hipMemcpyAsync(d_own_e.p, own_e.data(), sizeof(int32_t) * ne, hipMemcpyHostToDevice, s), "tiles");
   HIP_OK(hipMemsetAsync(d_status.p, 0, sizeof(int32_t), s), "tiles");
-  std::vector<int32_t> tv2, counts, leftover;
-  const MeshView mv = m->view();
-  for (int cap = 2048; cap >= 32; cap /= 2) {
-    tv2.clear();
-    leftover.clear();
-    for (int64_t v = 0; v < nv;) {
-      if (vslots[v] > TILE_B || vcount[v] > cap) {  // fitted outside the tiles
-        leftover.push_back((int32_t)v);
-        for (int32_t i = own_off[v]; i < own_off[v + 1]; ++i) leftover.push_back((int32_t)(nv + own_e[i]));
-        ++v;
-        continue;
-      }
-      const int64_t v0 = v;
-      int sl = 0, sum = 0;
-      while (v < nv && vslots[v] <= TILE_B && vcount[v] <= cap && sl + vslots[v] <= TILE_B &&
-             sum + vcount[v] <= cap) {
-        sl += vslots[v];
-        sum += vcount[v];
-        ++v;
-      }
-      tv2.push_back((int32_t)v0);
-      tv2.push_back((int32_t)v);
+  // A tile takes patches in vertex order (a vertex's patch, then the edges
+  // it owns) while they fit TILE_B lanes, the union of the v2t rows of the
+  // vertices they need holds at most TILE_MAX_UNION tets and those rows at
+  // most TILE_SORT_CAP entries; it may end inside a vertex's edges, the next
+  // tile taking the rest.  The unions are followed on the host (the rows
+  // are ascending tet ids, as the device CSR's).  Per tile (TILE_DESC
+  // ints): candidate rows [cv0, cv1), vertex patches [pv0, pv1), owned
+  // edges own_e[pe0, pe1).
+  std::vector<int32_t> hoff(nv + 1, 0), hrow((size_t)4 * nt);
+  for (int64_t v = 0; v < nv; ++v) hoff[v + 1] = hoff[v] + vcount[v];
+  {
+    std::vector<int32_t> cur(hoff.begin(), hoff.end() - 1);
+    for (int64_t t = 0; t < nt; ++t)
+      for (int k = 0; k < 4; ++k) hrow[cur[tet_verts[4 * t + k]]++] = (int32_t)t;
+  }
+  std::vector<int32_t> desc, counts, leftover, uni, merged;
+  int32_t d[TILE_DESC] = {0, 0, 0, 0, 0, 0};
+  bool open = false;
+  int sl = 0, cand = 0;
+  auto close = [&]() {
+    if (open) desc.insert(desc.end(), d, d + TILE_DESC);
+    open = false;
+  };
+  auto start = [&](int32_t v) {
+    d[0] = d[1] = d[2] = d[3] = v;
+    d[4] = d[5] = own_off[v];
+    sl = cand = 0;
+    uni.clear();
+    open = true;
+  };
+  auto row = [&](int32_t v) { return hrow.begin() + hoff[v]; };
+  // an item of vertex v taking `lanes` lanes: the tile in progress takes it
+  // if it fits, else a new tile starts with it
+  auto take = [&](int32_t v, int lanes) {
+    if (!open) start(v);
+    bool fits = sl + lanes <= TILE_B;
+    if (fits && d[1] <= v) {  // v's row would join the candidates
+      merged.clear();
+      std::set_union(uni.begin(), uni.end(), row(v), row(v + 1), std::back_inserter(merged));
+      fits = (int)merged.size() <= TILE_MAX_UNION && cand + vcount[v] <= TILE_SORT_CAP;
     }
-    const int64_t ntile = (int64_t)tv2.size() / 2;
-    counts.assign(ntile + 1, 0);
-    if (!ntile) break;
-    if ((rc = d_vert.ensure(sizeof(int32_t) * 2 * ntile)) || (rc = m->t_tet_off.ensure(sizeof(int32_t) * (ntile + 1))))
-      return rc;
-    HIP_OK(hipMemcpyAsync(d_vert.p, tv2.data(), sizeof(int32_t) * 2 * ntile, hipMemcpyHostToDevice, s), "tiles");
-    if ((rc = launch_tile_build(mv, ntile, d_vert.as<int32_t>(), nullptr, nullptr, 0, m->t_tet_off.as<int32_t>(),
-                                nullptr, nullptr, nullptr, d_status.as<int32_t>(), s)))
-      return rc;
-    HIP_OK(hipMemcpyAsync(counts.data() + 1, m->t_tet_off.as<int32_t>() + 1, sizeof(int32_t) * ntile,
-                          hipMemcpyDeviceToHost, s), "tiles");
-    HIP_OK(hipStreamSynchronize(s), "tiles");
-    int mx = 0;
-    for (int64_t i = 1; i <= ntile; ++i) mx = std::max(mx, counts[i]);
-    if (mx <= TILE_MAX_UNION || cap == 32) {
-      if (mx > TILE_MAX_UNION) {  // give up tiling: every patch fits thread per patch
-        tv2.clear();
-        counts.assign(1, 0);
-        leftover.clear();
-      }
-      m->max_union = mx <= TILE_MAX_UNION ? mx : 0;
-      break;
+    if (!fits) {
+      close();
+      start(v);
+      merged.assign(row(v), row(v + 1));
+    }
+    if (d[1] <= v) {
+      uni.swap(merged);
+      d[1] = v + 1;
+      cand += vcount[v];
+    }
+    sl += lanes;
+  };
+  for (int32_t v = 0; v < (int32_t)nv; ++v) {
+    if (vcount[v] > TILE_MAX_UNION) {  // fitted outside the tiles, with its edges
+      close();
+      leftover.push_back(v);
+      for (int32_t i = own_off[v]; i < own_off[v + 1]; ++i) leftover.push_back((int32_t)(nv + own_e[i]));
+      continue;
+    }
+    take(v, 1 << group_log2(vcount[v]));
+    if (d[2] == d[3]) d[2] = v;
+    d[3] = v + 1;
+    for (int32_t i = own_off[v]; i < own_off[v + 1]; ++i) {
+      take(v, 1 << group_log2(ecount[own_e[i]]));
+      if (d[4] == d[5]) d[4] = i;
+      d[5] = i + 1;
     }
   }
-  const int64_t ntile = (int64_t)tv2.size() / 2;
+  close();
+  std::vector<int32_t>().swap(hrow);
+  const MeshView mv = m->view();
+  {
+    // the union sizes, from the device build (the same unions)
+    const int64_t ntile = (int64_t)desc.size() / TILE_DESC;
+    counts.assign(ntile + 1, 0);
+    if (ntile) {
+      if ((rc = d_desc.ensure(sizeof(int32_t) * TILE_DESC * ntile)) ||
+          (rc = m->t_tet_off.ensure(sizeof(int32_t) * (ntile + 1))))
+        return rc;
+      HIP_OK(hipMemcpyAsync(d_desc.p, desc.data(), sizeof(int32_t) * TILE_DESC * ntile, hipMemcpyHostToDevice, s),
+             "tiles");
+      if ((rc = launch_tile_build(mv, ntile, d_desc.as<int32_t>(), nullptr, 0, m->t_tet_off.as<int32_t>(), nullptr,
+                                  nullptr, nullptr, d_status.as<int32_t>(), s)))
+        return rc;
+      HIP_OK(hipMemcpyAsync(counts.data() + 1, m->t_tet_off.as<int32_t>() + 1, sizeof(int32_t) * ntile,
+                            hipMemcpyDeviceToHost, s), "tiles");
+      HIP_OK(hipStreamSynchronize(s), "tiles");
+    }
+    int mx = 0;
+    for (int64_t i = 1; i <= ntile; ++i) mx = std::max(mx, counts[i]);
+    if (mx > TILE_MAX_UNION) {  // not expected: every patch fits thread per patch
+      desc.clear();
+      counts.assign(1, 0);
+      leftover.clear();
+    }
+    m->max_union = mx <= TILE_MAX_UNION ? mx : 0;
+  }
+  const int64_t ntile = (int64_t)desc.size() / TILE_DESC;
   m->ntile = ntile;
   m->nleftover = 0;
   if (!ntile) return 0;
@@ -362,11 +406,9 @@ int build_tiles(sprg_mesh* m, const int32_t* tet_verts, const int32_t* tet_edges
   std::vector<std::pair<int, int32_t>> pat;
   for (int64_t k = 0; k < ntile; ++k) {
     pat.clear();
-    for (int32_t v = tv2[2 * k]; v < tv2[2 * k + 1]; ++v) {
-      pat.push_back({group_log2(vcount[v]), v});
-      for (int32_t i = own_off[v]; i < own_off[v + 1]; ++i)
-        pat.push_back({group_log2(ecount[own_e[i]]), (int32_t)(nv + own_e[i])});
-    }
+    const int32_t* dk = desc.data() + TILE_DESC * k;
+    for (int32_t v = dk[2]; v < dk[3]; ++v) pat.push_back({group_log2(vcount[v]), v});
+    for (int32_t i = dk[4]; i < dk[5]; ++i) pat.push_back({group_log2(ecount[own_e[i]]), (int32_t)(nv + own_e[i])});
     std::stable_sort(pat.begin(), pat.end(),
                      [](const std::pair<int, int32_t>& x, const std::pair<int, int32_t>& y) { return x.first > y.first; });
     for (const auto& pr : pat) {
@@ -395,8 +437,8 @@ int build_tiles(sprg_mesh* m, const int32_t* tet_verts, const int32_t* tet_edges
   if (!leftover.empty())
     HIP_OK(hipMemcpyAsync(m->t_leftover.p, leftover.data(), sizeof(int32_t) * leftover.size(), hipMemcpyHostToDevice, s),
            "tiles");
-  if ((rc = launch_tile_build(mv, ntile, d_vert.as<int32_t>(), edges ? d_own_off.as<int32_t>() : nullptr,
-                              d_own_e.as<int32_t>(), 1, m->t_tet_off.as<int32_t>(), m->t_tets.as<int32_t>(),
+  if ((rc = launch_tile_build(mv, ntile, d_desc.as<int32_t>(), edges ? d_own_e.as<int32_t>() : nullptr, 1,
+                              m->t_tet_off.as<int32_t>(), m->t_tets.as<int32_t>(),
                               m->t_loc.as<uint16_t>(), m->t_loc.as<uint16_t>() + 4 * nt, d_status.as<int32_t>(),
                               s)))
     return rc;
