@@ -90,9 +90,10 @@ struct LaunchShape {
 };
 
 // Tiles of the regular fit pass (built once per mesh, k_tile.hip).  A tile
-// is a run of consecutive vertices with the edges they own (an edge belongs
-// to its lower vertex; its tets are among that vertex's tets), so the union
-// of the tile's vertex patches holds every initial set of the tile.  Each
+// takes patches in vertex order -- a vertex's patch, then the edges it owns
+// (an edge belongs to its lower vertex; its tets are among that vertex's
+// tets) -- and may end inside a vertex's edges, so the union of the tets of
+// the vertices it touches holds every initial set of the tile.  Each
 // patch is fitted by a group of g = 1, 2, 4 or 8 lanes (by set size), and
 // the groups of a tile are laid out in descending size from lane 0, so a
 // group never straddles a wave and starts at a multiple of its size.

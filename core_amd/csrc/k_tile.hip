@@ -8,11 +8,12 @@
 // vertex ids, coordinates and sample points about ten times, through chains
 // of dependent loads (set -> vertex ids -> coordinates).
 //
-// Here a block takes a tile: a run of consecutive vertices with the edges
-// they own (an edge's tets all contain its lower vertex, so they lie in that
-// vertex's patch).  The block loads the union of the tile's vertex patches
-// once, with coalesced loads, computes the tets' sample points into LDS, and
-// then fits every patch of the tile from LDS:
+// Here a block takes a tile: a run of patches in vertex order, each vertex's
+// patch followed by the edges it owns (an edge's tets all contain its lower
+// vertex, so they lie in that vertex's patch), possibly ending inside a
+// vertex's edges.  The block loads the union of the tets of the vertices
+// the tile touches once, with coalesced loads, computes the tets' sample
+// points into LDS, and then fits every patch of the tile from LDS:
 //
 //   - a patch is fitted by a group of g = 1, 2, 4 or 8 lanes (chosen by its
 //     tet count when the tiles were built), lane r of the group taking the
