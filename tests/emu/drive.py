@@ -112,6 +112,30 @@ def exact_sequence(ctx, name, h, shapes=True, fast=True):
     L.sprg_mesh_destroy(m)
 
 
+def leftover_check(ctx):
+    # a vertex with more tets than a tile union holds: its patch is fitted
+    # outside the tiles, by the thread-per-patch pass (one point per tet, so
+    # the exact form takes it too)
+    tv, xyz = smb_util.star(70)
+    h = mesh.HostMesh(tv, xyz, 1)
+    check("star(70) has a vertex of more than 128 tets", np.bincount(h.tv.ravel()).max() > 128)
+    om = O.Mesh(h.tv, h.xyz, 1)
+    m = device_mesh(ctx, h)
+    ip = np.ascontiguousarray(workloads.strain_field(h.ip_points(1)))
+    ref = O.recover(om, ip)
+    scale = np.abs(ref).max(0)
+    keep = scale > 0
+    for form in (0, 1, 2):
+        out = np.full((h.nv, 9), np.nan)
+        _lib.check(L.sprg_recover(m, 1, 9, P(ip), P(out), form))
+        if form == 0:
+            ok = np.array_equal(out, ref)
+        else:
+            ok = np.max(np.abs(out - ref)[:, keep] / scale[keep]) <= 1e-9
+        check("leftover vertex recovered, form %d" % form, ok)
+    L.sprg_mesh_destroy(m)
+
+
 def batch_check(ctx):
     # two P1 meshes in one recorded graph, first with the target estimator,
     # then with the adapt-ratio estimator; between replays, eager calls
@@ -179,8 +203,9 @@ def main():
     ctx = vp()
     _lib.check(L.sprg_ctx_create(0, ctypes.byref(ctx)))
     t0 = time.time()
-    if sys.argv[1:] == ["batch"]:  # the batch check alone
-        batch_check(ctx)
+    only = {"batch": batch_check, "leftover": leftover_check}
+    if len(sys.argv) == 2 and sys.argv[1] in only:  # one check alone
+        only[sys.argv[1]](ctx)
         print("ALL PASS (%.0f s)" % (time.time() - t0), flush=True)
         return
 
@@ -201,6 +226,7 @@ def main():
     # in every form: the sequence of the GPU tests held after an unexplained
     # fault
     timed("p1 n=4", exact_sequence, ctx, "p1 n=4", mesh.box(4, order=1), shapes=False)
+    timed("leftover", leftover_check, ctx)
     timed("batch", batch_check, ctx)
     print("ALL PASS (%.0f s)" % (time.time() - t0), flush=True)
 

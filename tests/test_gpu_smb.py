@@ -18,7 +18,7 @@ import os
 import numpy as np
 import pytest
 
-from core_amd import _lib, spr, workloads
+from core_amd import _lib, mesh, spr, workloads
 from oracle import oracle as O
 import smb_util
 
@@ -110,4 +110,26 @@ def test_delaunay_large_sets_take_the_large_kernel(gpu, delaunay_path):
     r = spr.recoverField(f)
     assert m.last_stats()["large_sets"] > 0
     m.destroyField(r)
+    m.close()
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32r", "fp64f"])
+def test_vertex_beyond_a_tile_union_is_fitted_outside_the_tiles(gpu, form):
+    # the centre of star(70) has about 136 tets, more than a tile's union
+    # holds (128): its patch takes the thread-per-patch pass in the fast
+    # forms; at one point per tet the exact form fits it as well
+    tv, xyz = smb_util.star(70)
+    h = mesh.HostMesh(tv, xyz, 1)
+    assert np.bincount(h.tv.ravel()).max() > 128
+    m = spr.Mesh(h)
+    ip = workloads.strain_field(h.ip_points(1))
+    f = m.createIPField("eps", 9, 1, ip)
+    got = spr.recoverField(f, form=form).numpy()
+    ref = O.recover(_omesh(h), ip)
+    if form == "fp64":
+        assert np.array_equal(got, ref)
+    else:
+        scale = np.abs(ref).max(0)
+        keep = scale > 0
+        assert np.max(np.abs(got - ref)[:, keep] / scale[keep]) <= 1e-9
     m.close()
