@@ -240,32 +240,45 @@ __device__ __forceinline__ void stage_tets(const MeshView& m, const TileNext& n,
                                            double* sX)
 {
   const int l = threadIdx.x;
-  for (int k = 0; k < 2; ++k) {
-    const int u = l + k * TILE_B;
-    if (u >= n.U) break;
-    const int32_t t = k == 0 ? n.ta : n.tb;
-    sTet[u] = t;
-    if constexpr (P2MAP) {
+  if constexpr (P2MAP) {
+    for (int k = 0; k < 2; ++k) {
+      const int u = l + k * TILE_B;
+      if (u >= n.U) break;
+      const int32_t t = k == 0 ? n.ta : n.tb;
+      sTet[u] = t;
       for (int r = 0; r < q; ++r) {
         double x[3];
         sample_x<true>(m, t, r, x);
         for (int j = 0; j < 3; ++j) sX[3 * (u * q + r) + j] = x[j];
       }
-    } else {
+    }
+  } else {
+    // both tets' vertex coordinates in flight before either is mapped (the
+    // vertex ids of a lane without a second tet are an earlier tile's, or
+    // zero: valid vertices, so the loads need no guard)
+    double P[2][4][3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
       const int4 v = k == 0 ? n.va : n.vb;
       const int32_t ids[4] = {v.x, v.y, v.z, v.w};
-      double P[4][3];
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) P[a][j] = m.xyz[3 * (int64_t)ids[a] + j];
+        for (int j = 0; j < 3; ++j) P[k][a][j] = m.xyz[3 * (int64_t)ids[a] + j];
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int u = l + k * TILE_B;
+      if (u >= n.U) break;
+      sTet[u] = k == 0 ? n.ta : n.tb;
       for (int r = 0; r < q; ++r) {
         double xi[3];
         rule_point(m.order, r, xi);
         const double l0 = 1 - xi[0] - xi[1] - xi[2];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-          sX[3 * (u * q + r) + j] = fma(xi[2], P[3][j], fma(xi[1], P[2][j], fma(xi[0], P[1][j], l0 * P[0][j])));
+          sX[3 * (u * q + r) + j] =
+              fma(xi[2], P[k][3][j], fma(xi[1], P[k][2][j], fma(xi[0], P[k][1][j], l0 * P[k][0][j])));
       }
     }
   }
