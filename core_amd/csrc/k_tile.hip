@@ -582,43 +582,64 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
     double v[NC], b0[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[c] = 0;
-    if (solve) {
-      const double* bp = a.ip + (int64_t)sTet[loc[0]] * q * NC;
+    auto accumulate = [&](const double* X, const double* b) {
+      const double d[3] = {X[0] - xn[0], X[1] - xn[1], X[2] - xn[2]};
+      double f[NT];
+      basis<double, NT>(d, f);
+      double w = 0;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) b0[c] = bp[c];
-      auto accumulate = [&](const double* X, const double* b) {
-        const double d[3] = {X[0] - xn[0], X[1] - xn[1], X[2] - xn[2]};
-        double f[NT];
-        basis<double, NT>(d, f);
-        double w = 0;
+      for (int k = 0; k < NT; ++k) w = fma(f[k], z[k], w);
 #pragma unroll
-        for (int k = 0; k < NT; ++k) w = fma(f[k], z[k], w);
+      for (int c = 0; c < NC; ++c) v[c] = fma(w, b[c] - b0[c], v[c]);
+    };
+    if (q == 4 && 4 * NC <= 36) {
+      // the tet's whole q x nc block in flight at once (G is dead here).
+      // The set's first sample b0 is the first point of the first tet of
+      // the group's rank-0 lane: every lane loads its first tet's block,
+      // and the group takes b0 from that lane (every lane of the wave takes
+      // part in the exchange; a group is all solved or all not)
+      double b[4 * NC];
+      const int uf = solve ? loc[rank] : 0;
+      int un = solve && rank + g < ns ? loc[rank + g] : 0;  // in flight with the first block
+      if (solve) {
+        const double* bt = a.ip + (int64_t)sTet[uf] * 4 * NC;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) v[c] = fma(w, b[c] - b0[c], v[c]);
-      };
-      if (q == 4 && 4 * NC <= 36) {
-        // the tet's whole q x nc block in flight at once (G is dead here)
-        for_lane_tets(loc, rank, g, ns, [&](int u) {
+        for (int i = 0; i < 4 * NC; ++i) b[i] = bt[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4 * NC; ++i) b[i] = 0;
+      }
+      const int src = (int)threadIdx.x - rank;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) b0[c] = __shfl(b[c], src);
+      if (solve) {
+#pragma unroll
+        for (int l = 0; l < 4; ++l) accumulate(sX + 12 * uf + 3 * l, b + l * NC);
+        for (int s = rank + g; s < ns; s += g) {
+          const int u = un;
+          un = loc[s + g < ns ? s + g : ns - 1];
           const double* X = sX + 12 * u;
           const double* bt = a.ip + (int64_t)sTet[u] * 4 * NC;
-          double b[4 * NC];
 #pragma unroll
           for (int i = 0; i < 4 * NC; ++i) b[i] = bt[i];
 #pragma unroll
           for (int l = 0; l < 4; ++l) accumulate(X + 3 * l, b + l * NC);
-        });
-      } else {
-        for_lane_tets(loc, rank, g, ns, [&](int u) {
-          const double* X = sX + 3 * q * u;
-          const double* bt = a.ip + (int64_t)sTet[u] * q * NC;
-          for (int l = 0; l < q; ++l) {
-            double b[NC];
-#pragma unroll
-            for (int c = 0; c < NC; ++c) b[c] = bt[l * NC + c];
-            accumulate(X + 3 * l, b);
-          }
-        });
+        }
       }
+    } else if (solve) {
+      const double* bp = a.ip + (int64_t)sTet[loc[0]] * q * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) b0[c] = bp[c];
+      for_lane_tets(loc, rank, g, ns, [&](int u) {
+        const double* X = sX + 3 * q * u;
+        const double* bt = a.ip + (int64_t)sTet[u] * q * NC;
+        for (int l = 0; l < q; ++l) {
+          double b[NC];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) b[c] = bt[l * NC + c];
+          accumulate(X + 3 * l, b);
+        }
+      });
     }
     group_sum<double, NC>(v, g, gmax);
     if (solve && rank == 0) {
