@@ -415,9 +415,6 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
   next_verts(m, nx);
   for (int64_t tile = w.first; tile < w.end; tile += w.step) {
     const TileNext cur = nx;
-    stage_tets<P2MAP>(m, cur, q, sTet, sX);
-    next_meta(tv, tile + w.step < w.end ? tile + w.step : tv.ntile, nx);  // arrives during the Gram sums
-    __syncthreads();
     // this lane's slot: patch, group size, rank in the group, set
     const bool has = (int)threadIdx.x < cur.nslot;
     const uint32_t sv = has ? (uint32_t)cur.sl.x : 0u;
@@ -426,17 +423,21 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
     const int rank = (int)(sv >> 29);
     const uint16_t* const loc = tv.loc + (has ? cur.sl.y : 0);
     const int ns = has ? cur.sl.z : 0;
-    const uint64_t b2 = __ballot(g >= 2), b4 = __ballot(g >= 4), b8 = __ballot(g >= 8);
-    const int gmax = b8 ? 8 : b4 ? 4 : b2 ? 2 : 1;
     // fewer samples than terms (hasEnoughPoints): straight to expansion
     const bool short_set = has && ns * q < NT;
     const bool work = has && !short_set;
+    // the patch's node, loaded while the tile is staged
     double xn[3] = {0, 0, 0};
     if (work) {
       xn[0] = m.xyz[3 * p];
       xn[1] = m.xyz[3 * p + 1];
       xn[2] = m.xyz[3 * p + 2];
     }
+    stage_tets<P2MAP>(m, cur, q, sTet, sX);
+    next_meta(tv, tile + w.step < w.end ? tile + w.step : tv.ntile, nx);  // arrives during the Gram sums
+    __syncthreads();
+    const uint64_t b2 = __ballot(g >= 2), b4 = __ballot(g >= 4), b8 = __ballot(g >= 8);
+    const int gmax = b8 ? 8 : b4 ? 4 : b2 ? 2 : 1;
     // Gram matrix of the group's tets, packed lower triangle
     T G[NG];
     if constexpr (NT == 10) {
