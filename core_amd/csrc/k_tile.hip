@@ -598,7 +598,9 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
       // The set's first sample b0 is the first point of the first tet of
       // the group's rank-0 lane: every lane loads its first tet's block,
       // and the group takes b0 from that lane (every lane of the wave takes
-      // part in the exchange; a group is all solved or all not)
+      // part in the exchange; a group is all solved or all not).  Every lane
+      // of a group has a first tet: the tile builder gives a set of ns tets
+      // g <= ns lanes (group_log2 in runtime.hip), so rank < ns here.
       double b[4 * NC];
       const int uf = solve ? loc[rank] : 0;
       int un = solve && rank + g < ns ? loc[rank + g] : 0;  // in flight with the first block

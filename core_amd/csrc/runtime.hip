@@ -243,7 +243,8 @@ extern "C" int sprg_download(sprg_ctx* ctx, void* dst, const void* src, size_t b
 namespace {
 
 // log2 of the lanes that fit a patch of ns tets in the tiled pass: about
-// six tets per lane, at most eight lanes
+// six tets per lane, at most eight lanes, and never more lanes than tets
+// (k_tile's weighted sum gives every lane of a solved set a first tet)
 int group_log2(int64_t ns) { return ns <= 6 ? 0 : ns <= 12 ? 1 : ns <= 24 ? 2 : 3; }
 
 // Builds the tiles of the regular fast pass (k_tile.hip) from the host
