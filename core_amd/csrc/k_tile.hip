@@ -569,14 +569,17 @@ __global__ __launch_bounds__(TILE_B) void k_tile(FitArgs a, float tau_rank, floa
       }
       if (refine && !(last_dz <= 1e-13 * zmax)) act = poor;
     }
-    // appends by the group's first lane
+    // appends by the group's first lane; one ballot skips all nine for a
+    // wave with nothing to append (most tiles)
     const int app = rank == 0 ? act : ACT_NONE;
-    wave_append(ACT_EXACT, app, a.L.count + 2 * NBUCKET, a.L.xq,
-                short_set ? (int32_t)p : (int32_t)(p | 0x80000000u));
-    wave_append(ACT_FALLBACK, app, a.L.count + 2 * NBUCKET + 1, a.L.fbq, (int32_t)p);
-    for (int b = 0; b < NBUCKET; ++b)
-      wave_append(b, app == ACT_REFIT ? bucket : -1, a.L.count + b, a.L.items[b], (int32_t)p);
-    big_append(app == ACT_BIG, a.L, (int32_t)p);
+    if (__ballot(app != ACT_SOLVE && app != ACT_NONE)) {  // wave-uniform
+      wave_append(ACT_EXACT, app, a.L.count + 2 * NBUCKET, a.L.xq,
+                  short_set ? (int32_t)p : (int32_t)(p | 0x80000000u));
+      wave_append(ACT_FALLBACK, app, a.L.count + 2 * NBUCKET + 1, a.L.fbq, (int32_t)p);
+      for (int b = 0; b < NBUCKET; ++b)
+        wave_append(b, app == ACT_REFIT ? bucket : -1, a.L.count + b, a.L.items[b], (int32_t)p);
+      big_append(app == ACT_BIG, a.L, (int32_t)p);
+    }
     next_verts(m, nx);  // arrives during the weighted sum
     // weighted sum over the lane's tets, centred on the set's first sample
     const bool solve = act == ACT_SOLVE;
