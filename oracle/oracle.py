@@ -193,6 +193,21 @@ def target_size(mesh, ip_order, ip, target, alpha=0.25, beta=2.0):
     return size, diag
 
 
+def target_size_rec(mesh, ip_order, ip, rec, target, alpha=0.25, beta=2.0):
+    """getTargetSPRSizeField's estimator stage from a given recovered field."""
+    ip = np.ascontiguousarray(ip, dtype=np.float64)
+    rec = np.ascontiguousarray(rec, dtype=np.float64)
+    nc = ip.shape[2]
+    size = np.zeros(mesh.nv)
+    diag = np.zeros(4)
+    f = lib().om_target_size_rec
+    f.argtypes = [_p, ctypes.c_int, ctypes.c_int, _p, _p, ctypes.c_double, ctypes.c_double,
+                  ctypes.c_double, _p, _p]
+    _check(f(mesh.c(), ip_order, nc, _ptr(ip), _ptr(rec), float(target), alpha, beta,
+             _ptr(size), _ptr(diag)))
+    return size, diag
+
+
 def element_error(mesh, ip_order, ip, rec):
     ip = np.ascontiguousarray(ip, dtype=np.float64)
     rec = np.ascontiguousarray(rec, dtype=np.float64)

@@ -97,7 +97,9 @@ def test_strain120(gpu):
     oracle then runs over all 10,368,000 tets on that recovered field, and
     the GPU size fields are held to it: the FP64 form to 1e-12 (summation
     order of the two global sums and of the vertex averages aside, it
-    computes the same numbers), the fast forms to 1e-6."""
+    computes the same numbers), the fast forms to 1e-6.  The same for
+    getTargetSPRSizeField aiming at as many tets as the mesh has (its
+    alpha/beta clamps active on part of the mesh)."""
     h = workloads.strain_mesh(120)
     assert (h.nt, h.nv, h.ne) == (10368000, 1771561, 12225960)
     ip = workloads.strain_ip(h)
@@ -124,3 +126,14 @@ def test_strain120(gpu):
         m.destroyField(s)
         print("strain_120 %s size vs oracle: max %.3e rms %.3e" % (form, err.max(), np.sqrt((err ** 2).mean())))
         assert err.max() <= 1e-6
+    tref, tdiag = O.target_size_rec(om, 2, ip, got, float(h.nt))
+    assert np.all(tref > 0) and np.all(np.isfinite(tref))
+    for form, tol in (("fp64", 1e-12), ("fp64f", 1e-6)):
+        diag = {}
+        s = spr.getTargetSPRSizeField(eps, h.nt, form=form, diag=diag)
+        err = np.abs(s.numpy()[:, 0] / tref - 1)
+        m.destroyField(s)
+        print("strain_120 %s target size vs oracle: max %.3e rms %.3e (G %.9e / %.9e)"
+              % (form, err.max(), np.sqrt((err ** 2).mean()), diag["global_error"], tdiag[0]))
+        assert abs(diag["global_error"] / tdiag[0] - 1) <= tol
+        assert err.max() <= tol

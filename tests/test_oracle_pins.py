@@ -396,6 +396,9 @@ def test_target_size_formulas(order):
         high = high or bool(np.any(raw > beta * L))
         assert abs(diag[1] / sf - 1) < 1e-12
         assert np.abs(size / _vertex_mean(h, hn) - 1).max() < 1e-11
+        # the estimator stage alone, from the same recovered field
+        size_r, diag_r = O.target_size_rec(om, 2, ip, rec, N, alpha, beta)
+        assert np.array_equal(size_r, size) and diag_r[1] == diag[1]
     assert low and high  # both bounds were exercised
 
 
