@@ -99,7 +99,8 @@ def test_strain120(gpu):
     order of the two global sums and of the vertex averages aside, it
     computes the same numbers), the fast forms to 1e-6.  The same for
     getTargetSPRSizeField aiming at as many tets as the mesh has (its
-    alpha/beta clamps active on part of the mesh)."""
+    alpha/beta clamps active on part of the mesh); then getGradIPField of a
+    vector field on the same mesh, bit for bit on sampled tet ranges."""
     h = workloads.strain_mesh(120)
     assert (h.nt, h.nv, h.ne) == (10368000, 1771561, 12225960)
     ip = workloads.strain_ip(h)
@@ -137,3 +138,21 @@ def test_strain120(gpu):
               % (form, err.max(), np.sqrt((err ** 2).mean()), diag["global_error"], tdiag[0]))
         assert abs(diag["global_error"] / tdiag[0] - 1) <= tol
         assert err.max() <= tol
+    m.destroyField(eps)
+    # getGradIPField at this size: a Lagrange-2 vector field's IP gradients
+    # (10.4M tets x 4 points x 9 values; the last tets' outputs lie past 2^31
+    # bytes) bit for bit on tet ranges at the start, middle and end, the
+    # oracle running on the sub-mesh of each range (a tet's gradient depends
+    # on its own nodes only)
+    P = workloads.node_points(h)
+    u = np.stack([workloads.sin_field(P), workloads.exp_field(P), P[:, 0] * P[:, 1]], 1)
+    fu = m.createLagrangeField("u", 3, 2, u)
+    gu = spr.getGradIPField(fu, "gu", 2)
+    ggot = gu.numpy()
+    m.destroyField(gu)
+    m.destroyField(fu)
+    assert ggot.shape == (h.nt, 4, 9)
+    w = 100000
+    for t0 in (0, h.nt // 2, h.nt - w):
+        sub = O.Mesh(h.tv[t0:t0 + w], h.xyz, h.order, te=h.te[t0:t0 + w], ev=h.ev)
+        assert np.array_equal(ggot[t0:t0 + w], O.grad_ip(sub, 2, u, 2)), t0
