@@ -124,6 +124,87 @@ def cpu_baseline(host, ip, workers):
     return out, size, rec
 
 
+def grad_bytes(T, N, c_in=3, q=4, k=10):
+    """Compulsory HBM bytes of getGradIPField: nodal values and coordinates
+    of the N nodes, the T tets' k node ids, the T x q x 3 c_in gradients."""
+    return N * c_in * 8 + N * 24 + T * k * 4 + T * q * 3 * c_in * 8
+
+
+def grad_main(args, np, torch, spr, workloads):
+    """--op grad: spr::getGradIPField of a Lagrange-2 vector field (3
+    components -> 9 values at each of the 4 points of the order-2 rule) on
+    the same mesh; one step = one call with the nodal field resident in HBM;
+    value = tets per second of the call; roofline = the gradient kernel's
+    algorithmic bytes over its GPU time (HIP events the library records on
+    its stream); cpu_baseline = the FP64 oracle's gradient of the mesh's
+    first 2^20 tets on one core."""
+    t0 = time.time()
+    host = workloads.strain_mesh(args.n)
+    P = workloads.node_points(host)
+    u = np.stack([workloads.sin_field(P), workloads.exp_field(P), P[:, 0] * P[:, 1]], 1)
+    log("mesh n=%d: %d tets, %d nodes (%.1fs)" % (args.n, host.nt, P.shape[0], time.time() - t0))
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        w = min(host.nt, 1 << 20)
+        sub = O.Mesh(host.tv[:w], host.xyz, host.order, te=host.te[:w], ev=host.ev)
+        t1 = time.perf_counter()
+        O.grad_ip(sub, 2, u, 2)
+        dt = time.perf_counter() - t1
+        cpu = {"value": w / dt, "unit": "tets/s", "cores": 1, "kind": "port",
+               "sample": "the FP64 oracle's getGradIPField over the mesh's first %d tets "
+                         "(4 points, 9 values each) on one core, %.2f s" % (w, dt)}
+        log("cpu baseline: %s" % cpu)
+    probe = copy_probe(torch)
+    m = spr.Mesh(host)
+    fu = m.createLagrangeField("u", 3, 2, u)
+    torch.cuda.synchronize()
+
+    def step():
+        g = spr.getGradIPField(fu, "g", 2)  # returns after the kernel is complete
+        m.destroyField(g)
+        return m.phase_times()[0]
+
+    for i in range(args.warmup):
+        step()
+    kernel_ms = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        kernel_ms.append(step())
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t_start) * 1e3 / args.steps
+    kms = float(np.mean(kernel_ms))
+    T, N = host.nt, P.shape[0]
+    gb = grad_bytes(T, N)
+    achieved = gb / (kms / 1e3) / 1e9
+    out = {
+        "metric": "getGradIPField tets/s @10.4M tets (P2 vector field, IP order 2)",
+        "value": T / (ms_per_step / 1e3),
+        "unit": "tets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "strain_120 mesh: getGradIPField of a P2 vector field at IP order 2, "
+                               "10,368,000 tets, 3 -> 9 components", "tets": T, "nodes": N},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "measured_copy_gbs": probe["gbs"],
+                     "kernel": "k_grad, %d algorithmic bytes per launch, %.3f ms" % (gb, kms)},
+    }
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def traffic_of(form):
     """HBM bytes per launch of the fit pass from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE measurement (scaled by the copy probe), for the
@@ -156,6 +237,8 @@ def main():
                     help="grid cap of the tiled regular fit pass (0: thread per patch instead)")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run the 1 GiB device copy probe and exit")
+    ap.add_argument("--op", default="size", choices=["size", "grad"],
+                    help="size: getSPRSizeField (the default measurement); grad: getGradIPField")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: measures one GPU only; runs on several GPUs are not supported "
@@ -169,6 +252,8 @@ def main():
     if args.copy_probe_only:
         print(json.dumps({"copy_probe": copy_probe(torch)}), flush=True)
         return 0
+    if args.op == "grad":
+        return grad_main(args, np, torch, spr, workloads)
 
     t0 = time.time()
     host = workloads.strain_mesh(args.n)

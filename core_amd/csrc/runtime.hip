@@ -877,9 +877,13 @@ extern "C" int sprg_grad_ip(sprg_mesh* m, int field_order, int nc_in, const doub
   if (rule_npoints(ip_order) < 0)
     return sprg_fail(SPRG_EUNSUPPORTED, "getGradIPField: integration order %d (1..3 supported)", ip_order);
   HIP_OK(hipSetDevice(m->ctx->device), "hipSetDevice");
+  m->phase_last = 0;
+  m->mark(0);
   int rc = launch_grad(m->view(), field_order, nc_in, nodal, ip_order, ip, m->ctx->stream);
   if (rc) return rc;
+  m->mark(1);
   HIP_OK(hipStreamSynchronize(m->ctx->stream), "grad sync");
+  collect_phases(m);  // phase 0: the gradient kernel
   return 0;
 }
 

@@ -171,7 +171,8 @@ class Mesh:
 
     def phase_times(self):
         """GPU ms of the last call's phases: fit, expand, expanded fit,
-        element pass, factor + average (HIP events on the library stream)."""
+        element pass, factor + average (HIP events on the library stream);
+        after getGradIPField, the gradient kernel first and zeros."""
         ms = np.zeros(5)
         _lib.lib().sprg_mesh_phase_times(self.handle, ms.ctypes.data, 5)
         return ms

@@ -93,7 +93,8 @@ int sprg_mesh_counts(sprg_mesh* mesh, int64_t* counts);
 /* GPU time (ms, HIP events on the context stream) of the phases of the last
  * recover / size-field call on this mesh: [0] fit of the initial patch sets,
  * [1] patch expansion, [2] fit of the expanded sets, [3] element pass,
- * [4] size factor + vertex average (0 for phases the call did not run).
+ * [4] size factor + vertex average (0 for phases the call did not run);
+ * after a gradient call, [0] is the gradient kernel and the rest 0.
  * Writes min(n, 5) values and returns 5.  No reference counterpart (the
  * reference times the whole call, spr/sprEstimateError.cc:313-319). */
 int sprg_mesh_phase_times(sprg_mesh* mesh, double* ms, int n);
