@@ -1,9 +1,9 @@
 /*
  * sprgpu.h -- C ABI of the MI355X (gfx950) Superconvergent Patch Recovery
  * library.  It replaces the error-estimation path of SCOREC/core's `spr`
- * library (spr/spr.h:33-63) for one mesh part of tetrahedra; the apf
- * adapter that maps apf::Field* in/out onto these calls is given in
- * INTEGRATION.md.
+ * library (spr/spr.h:33-63) for one mesh part of tetrahedra; an apf
+ * adapter maps apf::Field* in/out onto these calls (the four spr::
+ * functions over one device mirror per mesh).
  *
  * Conventions
  *   - Every function returns 0 on success or an SPRG_E* code; the message
